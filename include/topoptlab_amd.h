@@ -1,0 +1,141 @@
+/*
+ * topoptlab_amd — C ABI of the MI355X (gfx950) SIMP finite-element hot path.
+ *
+ * All vector arguments are DEVICE pointers owned by the caller, contiguous
+ * float64, in the reference's own numbering: node id
+ * (iz*(nelx+1) + ix)*(nely+1) + iy, DOF = ndpn*node + component, element id
+ * (iz*nelx + ix)*nely + iy (2D: nely*ix + iy).  DOF vectors are full length
+ * (fixed DOFs included); on output fixed DOFs hold 0, on input they are
+ * ignored.  Handles own their internal workspaces.  Every call enqueues on
+ * the handle's HIP stream; calls that return host scalars synchronise it.
+ *
+ * Return codes: TL_OK, TL_EINVAL (bad argument; Python raises ValueError),
+ * TL_ENOCONV (iterative solve did not converge; RuntimeError, as
+ * topoptlab/solve_linsystem.py:349-351), TL_EHIP (HIP runtime error).
+ * tl_last_error() returns the message of the last failing call of the
+ * calling thread.
+ */
+#ifndef TOPOPTLAB_AMD_H
+#define TOPOPTLAB_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { TL_OK = 0, TL_EINVAL = 1, TL_ENOCONV = 2, TL_EHIP = 3 };
+enum { TL_PREC_FP64 = 0, TL_PREC_FP32 = 1, TL_PREC_BF16 = 2, TL_PREC_FP16 = 3 };
+enum { TL_PRECOND_NONE = 0, TL_PRECOND_JACOBI = 1, TL_PRECOND_GMG = 2 };
+
+typedef struct tl_problem tl_problem;
+typedef struct tl_filter tl_filter;
+
+/* Structured Q4 (ndim 2) / H8 (ndim 3) grid with unit elements. */
+typedef struct {
+  int32_t ndim;          /* 2 or 3 */
+  int32_t nelx, nely, nelz;  /* nelz ignored for ndim == 2 */
+  int32_t ndpn;          /* DOFs per node: 1 (Poisson) or ndim (elasticity) */
+  const double* KE;      /* HOST, (2^ndim*ndpn)^2 row-major element matrix */
+  double eps, penal;     /* modified SIMP: s = eps + (1-eps) x^penal */
+  const int64_t* fixed;  /* HOST, Dirichlet (zero) DOF indices */
+  int64_t nfixed;
+  void* stream;          /* hipStream_t; NULL = default stream */
+} tl_grid_desc;
+
+/* Replaces the set-up half of topoptlab/topology_optimization.py:264-371
+ * (KE, edofMat, iK/jK, BC arrays): the structured grid is never assembled. */
+int tl_create(const tl_grid_desc* desc, tl_problem** out);
+void tl_destroy(tl_problem* p);
+int tl_set_stream(tl_problem* p, void* stream);
+const char* tl_last_error(void);
+int64_t tl_ndof(const tl_problem* p);
+int64_t tl_nel(const tl_problem* p);
+
+/* Replaces topology_optimization.py:419-420 (scale = simp(xPhys); Kes).
+ * xPhys: device, nel float64. */
+int tl_set_density(tl_problem* p, const double* xPhys);
+
+/* y = K_ff u (fixed rows/cols removed), matrix-free.  Replaces the CSR
+ * product of topology_optimization.py:429-451 + linear_solvers.py:114,130.
+ * prec: TL_PREC_FP64 or TL_PREC_FP32 (inputs/outputs stay float64). */
+int tl_apply_K(tl_problem* p, const double* u, double* y, int32_t prec);
+
+/* Diagonal of K_ff (fixed DOFs: 0). */
+int tl_diag_K(tl_problem* p, double* d);
+
+typedef struct {
+  int32_t nlevels;       /* grid levels incl. the fine one (>= 2) */
+  double omega;          /* damped-Jacobi weight */
+  int32_t nu1, nu2;      /* pre-/post-smoothing sweeps */
+  int32_t precision;     /* V-cycle arithmetic: TL_PREC_FP64 or TL_PREC_FP32 */
+} tl_mg_opts;
+
+/* Galerkin GMG hierarchy for the current density (multigrid.py:89-223,
+ * gmg.py:34-224).  Must be called after each tl_set_density that should
+ * update the preconditioner. */
+int tl_setup_mg(tl_problem* p, const tl_mg_opts* opts);
+
+typedef struct {
+  double rtol, atol;     /* stop: ||r||_2 < max(atol, rtol*||b||_2) (linear_solvers.py:107-117) */
+  int32_t maxiter;
+  int32_t x0_given;      /* 1: u holds the initial guess (warm start) */
+  int32_t precond;       /* TL_PRECOND_* */
+  int32_t check_every;   /* iterations between host convergence checks */
+} tl_solve_opts;
+
+typedef struct {
+  int32_t iterations;
+  int32_t status;        /* 0 converged, TL_ENOCONV otherwise */
+  double relres_recursive;  /* ||r_k|| / ||b|| of the recursive residual */
+  double relres_true;       /* ||b - K_ff u|| / ||b||, recomputed in FP64 */
+} tl_solve_info;
+
+/* PCG on K_ff u = f (linear_solvers.py:50-136 via
+ * solve_linsystem.py:341-352).  f, u: device, ndof float64. */
+int tl_solve(tl_problem* p, const double* f, double* u,
+             const tl_solve_opts* opts, tl_solve_info* info);
+
+/* Compliance and sensitivities summed over ncases load cases
+ * (objectives.py:7-55, topology_optimization.py:463-505).
+ * u: device [ncases][ndof]; xPhys, dc: device nel; obj: HOST scalar. */
+int tl_compliance_sens(tl_problem* p, const double* u, int32_t ncases,
+                       const double* xPhys, double* obj, double* dc);
+
+/* Distance filter of filter/matrix_filter.py:101-232 applied as a stencil. */
+int tl_filter_create(int32_t ndim, int32_t nelx, int32_t nely, int32_t nelz,
+                     double rmin, void* stream, tl_filter** out);
+void tl_filter_destroy(tl_filter* f);
+int tl_filter_set_stream(tl_filter* f, void* stream);
+/* Hs (row sums of H): device nel. */
+int tl_filter_hs(tl_filter* f, double* Hs);
+/* mode 0: out = H in / Hs            (topology_optimization.py:640)
+ * mode 1: out = H (in / Hs)          (topology_optimization.py:557-558)
+ * mode 2: out = H(x*in)/Hs/max(1e-3,x) (topology_optimization.py:546-547)
+ * mode 3: out = H in                 (the product H @ v itself) */
+int tl_filter_apply(tl_filter* f, const double* in, double* out, int32_t mode,
+                    const double* x);
+
+/* Optimality-criteria bisection of optimizer/optimality_criterion.py:9-79
+ * (no element flags).  x, dc, dv, xnew: device n; g, gt: host scalars. */
+int tl_oc_update(int64_t n, const double* x, const double* dc,
+                 const double* dv, double g, double move, double l1, double l2,
+                 double* xnew, double* gt, void* stream);
+
+/* Instrumentation (no counterpart in the reference): while enabled, every
+ * launch of the fine-grid operator kernel is bracketed by HIP events on the
+ * handle's stream.  kind 0: FP64 operator (PCG product, true residual),
+ * kind 1: FP32/FP64 V-cycle operator (smoothing, residual).  tl_profile_read
+ * synchronises, returns the summed kernel time and launch count of a kind
+ * and clears that kind. */
+int tl_profile(tl_problem* p, int32_t enable);
+int tl_profile_read(tl_problem* p, int32_t kind, double* total_ms, int64_t* launches);
+
+/* mean(a) and max|a-b| of device vectors (topology_optimization.py:516,654). */
+int tl_mean_maxdiff(int64_t n, const double* a, const double* b, double* mean,
+                    double* maxdiff, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

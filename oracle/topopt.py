@@ -1,0 +1,69 @@
+"""Oracle: the reference's OC optimisation loop with a direct solve.
+
+Test infrastructure only (see ``oracle/__init__.py``).  Restates the body of
+``topoptlab/topology_optimization.py:main`` (loop at 411-673) for the branch
+``optimizer="oc"``, ``filter_mode="matrix"``, ``ft in {-1, 0, 1}``,
+``lin_solver_kw={"name": "scipy-direct"}``, compliance objective, SIMP.
+"""
+from __future__ import annotations
+
+import numpy as np
+from scipy.sparse.linalg import spsolve
+
+from . import fem
+
+
+def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
+                KE=None, eps=1e-9, penal=3.0, nouteriter=2000,
+                stop_on_change=True, history=None):
+    """Run the OC loop; return (x, obj) like ``topology_optimization.py:688``.
+
+    ``history`` (a list) receives one (obj, vol, change) tuple per iteration,
+    matching the log line of ``topology_optimization.py:669-670``.
+    ``stop_on_change=False`` disables the ``change < 0.01`` break (672-673)
+    so that a fixed number of iterations runs.
+    """
+    ndim = 2 if nelz is None else 3
+    n = int(np.prod([nelx, nely, nelz][:ndim]))
+    if KE is None:
+        KE = fem.ke_elast_2d() if ndim == 2 else fem.ke_elast_3d()
+    ndpn = KE.shape[-1] // 2 ** ndim
+    ndof = ndpn * int(np.prod(np.array([nelx, nely, nelz][:ndim]) + 1))
+    edof = (fem.edofmat_2d(nelx, nely, ndpn) if ndim == 2
+            else fem.edofmat_3d(nelx, nely, nelz, ndpn))
+    x = volfrac * np.ones(n)
+    xPhys = x.copy()
+    g = 0.0
+    H, Hs = fem.assemble_matrix_filter(nelx, nely, rmin, nelz)
+    Hs = Hs[:, 0]
+    if ndim == 2:
+        f, fixed, free = bcs(nelx, nely, ndof)
+    else:
+        f, fixed, free = bcs(nelx, nely, nelz, ndof)
+    u = np.zeros(f.shape)
+    xold = x.copy()
+    obj = 0.0
+    for loop in range(nouteriter):
+        scale = fem.simp(xPhys, eps, penal)
+        K = fem.apply_bc(fem.assemble_K(KE, edof, scale, ndof), free)
+        sol = spsolve(K, f[free])
+        u[free, :] = sol.reshape(len(free), -1)
+        obj, dc = fem.compliance_sens(KE, edof, u, xPhys, eps, penal)
+        dv = np.ones(n)
+        if ft == 0:
+            dc = (H @ (x * dc)) / Hs / np.maximum(0.001, x)
+        elif ft == 1:
+            dc = H @ (dc / Hs)
+            dv = H @ (dv / Hs)
+        xold = x.copy()
+        x, g = fem.oc_top88(x, volfrac, dc, dv, g)
+        if ft == 1:
+            xPhys = (H @ x) / Hs
+        else:
+            xPhys = x.copy()
+        change = np.abs(x - xold).max()
+        if history is not None:
+            history.append((obj, xPhys.mean(), change))
+        if stop_on_change and change < 0.01:
+            break
+    return x, obj

@@ -1,0 +1,236 @@
+"""GPU parity of the HIP path against the CPU oracle (marked ``gpu``).
+
+Tolerances: FP64 kernels 1e-12 relative (matrix-free summation order differs
+from scipy's CSR product); filter and OC kernels bit-exact (same summation
+order, no FMA contraction); FP32 operator 1e-5; solves compared after both
+sides converge to a tight residual.
+"""
+import sys
+
+import numpy as np
+import pytest
+from scipy.sparse.linalg import spsolve
+
+from oracle import fem, mg, topopt
+from tests.conftest import rel
+from topoptlab_amd import elements, example_bc, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _problem(dims, ndpn, seed=0, bc="cantilever"):
+    nelx, nely, nelz = dims
+    ndim = 2 if nelz is None else 3
+    if ndim == 2:
+        KE = fem.ke_elast_2d() if ndpn == 2 else fem.ke_poisson_2d()
+        edof = fem.edofmat_2d(nelx, nely, ndpn)
+    else:
+        KE = fem.ke_elast_3d() if ndpn == 3 else elements.lk_poisson_3d()
+        edof = fem.edofmat_3d(nelx, nely, nelz, ndpn)
+    nn = (nelx + 1) * (nely + 1) * (1 if nelz is None else nelz + 1)
+    ndof = ndpn * nn
+    # clamp the x = 0 face/edge (all components)
+    if ndim == 2:
+        nodes = np.arange(nely + 1)
+    else:
+        iz, iy = np.meshgrid(np.arange(nelz + 1), np.arange(nely + 1), indexing="ij")
+        nodes = (iz * (nelx + 1) * (nely + 1) + iy).ravel()
+    fixed = np.sort((ndpn * nodes[:, None] + np.arange(ndpn)[None, :]).ravel())
+    if bc == "none":
+        fixed = np.array([], dtype=np.int64)
+    free = np.setdiff1d(np.arange(ndof), fixed)
+    rng = np.random.default_rng(seed)
+    x = 0.001 + 0.999 * rng.random(edof.shape[0])
+    return KE, edof, ndof, fixed, free, x
+
+
+CASES = [((12, 8, None), 2), ((10, 6, None), 1), ((6, 4, 5), 3), ((5, 4, 3), 1),
+         ((24, 16, 20), 3), ((300, 7, None), 2)]
+
+
+@pytest.mark.parametrize("dims,ndpn", CASES)
+def test_ku_matches_assembled(gpu, dims, ndpn):
+    import torch
+    from topoptlab_amd.fem import StiffnessOperator
+    say("K.u", dims, ndpn)
+    KE, edof, ndof, fixed, free, x = _problem(dims, ndpn)
+    K = fem.apply_bc(fem.assemble_K(KE, edof, fem.simp(x), ndof), free)
+    op = StiffnessOperator(*dims, KE=KE, fixed=fixed, device=gpu)
+    op.set_density(x)
+    u = np.random.default_rng(1).standard_normal(len(free))
+    y_ref = K @ u
+    y = (op @ torch.as_tensor(u, device=gpu)).cpu().numpy()
+    assert rel(y, y_ref) < 1e-13
+    y32 = op.matvec_full(op.full(torch.as_tensor(u, device=gpu)), "fp32")[op._free_t].cpu().numpy()
+    assert rel(y32, y_ref) < 1e-5
+    d = op.diagonal().cpu().numpy()
+    assert rel(d, K.diagonal()) < 1e-14
+    # full-length input: fixed entries are ignored, fixed outputs are zero
+    uf = np.random.default_rng(2).standard_normal(ndof)
+    yf = op.matvec_full(torch.as_tensor(uf, device=gpu)).cpu().numpy()
+    assert np.all(yf[fixed] == 0)
+    assert rel(yf[free], K @ uf[free]) < 1e-13
+    op.close()
+
+
+@pytest.mark.parametrize("dims,ndpn,nlevels", [((16, 8, 8), 3, 3), ((32, 16, None), 2, 4),
+                                               ((16, 16, None), 1, 3), ((8, 4, 4), 3, 2)])
+def test_pcg_gmg_fp64_matches_oracle(gpu, dims, ndpn, nlevels):
+    """With an FP64 V-cycle the device preconditioner is the oracle's V(1,1)
+    (same Galerkin hierarchy, omega=0.6, zero coarse guesses, exact coarsest
+    solve), so PCG takes the same number of iterations."""
+    import torch
+    from topoptlab_amd.fem import StiffnessOperator
+    say("gmg fp64", dims, ndpn, nlevels)
+    KE, edof, ndof, fixed, free, _ = _problem(dims, ndpn)
+    x = workloads.gaussian_random_density(*dims[:2], dims[2], seed=5, sigma=2.0)
+    K = fem.apply_bc(fem.assemble_K(KE, edof, fem.simp(x), ndof), free).tocsr()
+    rng = np.random.default_rng(3)
+    b = rng.standard_normal(len(free))
+    h = mg.Hierarchy(K, *dims, ndpn, fixed, nlevels)
+    u_ref, info, it_ref = mg.pcg(K, b, h.vcycle, rtol=1e-10)
+    assert info == 0
+    op = StiffnessOperator(*dims, KE=KE, fixed=fixed, device=gpu)
+    op.set_density(x)
+    op.setup_mg(nlevels=nlevels, precision="fp64")
+    u, inf = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10)
+    u = u[op._free_t].cpu().numpy()
+    assert abs(inf["iterations"] - it_ref) <= 1, (inf, it_ref)
+    assert rel(u, u_ref) < 1e-7
+    ud = spsolve(K.tocsc(), b)
+    assert rel(u, ud) < 1e-6
+    assert inf["relres_true"] < 2e-10
+    # FP32 V-cycle converges to the same tolerance in a similar count
+    op.setup_mg(nlevels=nlevels, precision="fp32")
+    u32, inf32 = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10)
+    assert inf32["relres_true"] < 2e-10 and inf32["iterations"] <= 1.25 * it_ref + 2
+    # Jacobi-PCG reaches the same solution
+    uj, infj = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10,
+                        preconditioner="jacobi", maxiter=200000)
+    assert rel(uj[op._free_t].cpu().numpy(), ud) < 1e-6
+    op.close()
+
+
+@pytest.mark.parametrize("dims", [(60, 20, None), (13, 7, None), (10, 8, 6)])
+@pytest.mark.parametrize("rmin", [2.4, 1.2, 3.0])
+def test_filter_bitexact(gpu, dims, rmin):
+    import torch
+    from topoptlab_amd.filter import assemble_matrix_filter
+    say("filter", dims, rmin)
+    nelx, nely, nelz = dims
+    H, Hs = fem.assemble_matrix_filter(nelx, nely, rmin, nelz)
+    Hd, Hsd = assemble_matrix_filter(nelx, nely, rmin, nelz, device=gpu)
+    assert np.array_equal(Hsd.cpu().numpy(), Hs)
+    n = H.shape[0]
+    rng = np.random.default_rng(0)
+    x = rng.random(n)
+    dc = -rng.random(n)
+    xt = torch.as_tensor(x, device=gpu)
+    dct = torch.as_tensor(dc, device=gpu)
+    assert np.array_equal(Hd.apply(xt, 0).cpu().numpy(), H @ x / Hs[:, 0])
+    assert np.array_equal(Hd.apply(dct, 1).cpu().numpy(), H @ (dc / Hs[:, 0]))
+    assert np.array_equal((Hd @ xt).cpu().numpy(), H @ x)
+    ref2 = np.asarray((H @ (x * dc)) / Hs[:, 0]) / np.maximum(0.001, x)
+    assert np.array_equal(Hd.apply(dct, 2, x=xt).cpu().numpy(), ref2)
+
+
+@pytest.mark.parametrize("dims,ndpn,ncases", [((12, 8, None), 2, 2), ((6, 4, 5), 3, 1),
+                                              ((10, 6, None), 1, 1)])
+def test_compliance_sens(gpu, dims, ndpn, ncases):
+    import torch
+    from topoptlab_amd.fem import StiffnessOperator
+    say("compliance", dims, ndpn, ncases)
+    KE, edof, ndof, fixed, free, x = _problem(dims, ndpn)
+    u = np.random.default_rng(4).standard_normal((ndof, ncases))
+    obj_ref, dc_ref = fem.compliance_sens(KE, edof, u, x)
+    op = StiffnessOperator(*dims, KE=KE, fixed=fixed, device=gpu)
+    obj, dc = op.compliance_sens(torch.as_tensor(u.T.copy(), device=gpu), torch.as_tensor(x, device=gpu))
+    assert obj == pytest.approx(obj_ref, rel=1e-13)
+    assert rel(dc.cpu().numpy(), dc_ref) < 1e-13
+    op.close()
+
+
+@pytest.mark.parametrize("n,g", [(1200, 0.0), (5000, 0.3), (70000, -0.01)])
+def test_oc_bitexact(gpu, n, g):
+    import torch
+    from topoptlab_amd.optimizer import oc_top88
+    say("oc", n, g)
+    rng = np.random.default_rng(n)
+    x = rng.random(n)
+    dc = -rng.random(n) * 10 ** rng.uniform(-3, 1, n)
+    dv = 0.5 + rng.random(n)
+    xr, gr = fem.oc_top88(x, 0.5, dc, dv, g)
+    xd, gd = oc_top88(torch.as_tensor(x, device=gpu), 0.5, torch.as_tensor(dc, device=gpu),
+                      torch.as_tensor(dv, device=gpu), g)
+    assert np.array_equal(xd.cpu().numpy(), xr)
+    assert gd == pytest.approx(gr, rel=1e-12, abs=1e-12)
+
+
+def test_main_mbb_trajectory(gpu):
+    """Per-iteration objective of the device OC loop against the oracle's
+    scipy-direct loop: C1 (60x20 MBB, ft=1), 100 iterations, no early stop."""
+    from topoptlab_amd.topology_optimization import main
+    say("main mbb trajectory")
+    h_ref, h = [], []
+    topopt.topopt_loop(60, 20, 0.5, 2.4, ft=1, nouteriter=100, stop_on_change=False, history=h_ref)
+    main(60, 20, 0.5, 2.4, ft=1, nouteriter=100, stop_on_change=False, history=h,
+         lin_solver_kw={"name": "amd-gmgpcg", "rtol": 1e-12}, device=gpu)
+    objs = np.array([o for o, _, _ in h])
+    objs_ref = np.array([o for o, _, _ in h_ref])
+    err = np.abs(objs - objs_ref) / objs_ref
+    print("max rel obj error over 100 iterations:", err.max())
+    assert err.max() < 1e-8
+
+
+@pytest.mark.parametrize("ft,ref_idx", [(1, 1), (0, 0)])
+def test_main_mbb_golden(gpu, kats, ft, ref_idx):
+    """tests/test_compliance.py:8-35 on the device path (5 decimals)."""
+    from topoptlab_amd.topology_optimization import main
+    say("main mbb golden", ft)
+    ref = kats["compliance_filters"]["rows"][ref_idx][-1]
+    x, obj = main(60, 20, 0.5, 2.4, ft=ft, lin_solver_kw={"name": "amd-gmgpcg", "rtol": 1e-12},
+                  device=gpu)
+    assert x.shape == (1200, 1)
+    assert obj == pytest.approx(ref, abs=1.5e-5)
+
+
+def test_main_twoloads_golden(gpu, kats):
+    from topoptlab_amd.topology_optimization import main
+    say("main twoloads golden")
+    ref = kats["compliance_filters"]["rows"][4][-1]
+    _, obj = main(60, 20, 0.5, 2.4, ft=0, bcs=example_bc.cantilever_2d_twoloads,
+                  lin_solver_kw={"name": "amd-gmgpcg", "rtol": 1e-12}, device=gpu)
+    assert obj == pytest.approx(ref, abs=1.5e-5)
+
+
+def test_c2_size_operator_properties(gpu):
+    """192x96x96 H8 (5,447,811 DOF): symmetry, and rigid translations in the
+    null space of the unconstrained operator."""
+    import torch
+    from topoptlab_amd.fem import StiffnessOperator
+    say("c2 properties")
+    dims = (192, 96, 96)
+    x = workloads.gaussian_random_density(*dims, seed=0)
+    op = StiffnessOperator(*dims, KE=fem.ke_elast_3d(), fixed=[], device=gpu)
+    op.set_density(x)
+    g = torch.Generator(device=gpu).manual_seed(0)
+    u = torch.randn(op.ndof, dtype=torch.float64, device=gpu, generator=g)
+    v = torch.randn(op.ndof, dtype=torch.float64, device=gpu, generator=g)
+    a = torch.dot(v, op.matvec_full(u)).item()
+    b = torch.dot(u, op.matvec_full(v)).item()
+    assert abs(a - b) <= 1e-12 * (abs(a) + abs(b))
+    for d in range(3):
+        t = torch.zeros(op.ndof, dtype=torch.float64, device=gpu)
+        t[d::3] = 1.0
+        y = op.matvec_full(t)
+        assert y.abs().max().item() < 1e-13
+    # linearity of the fp32 path against the fp64 path
+    y64 = op.matvec_full(u)
+    y32 = op.matvec_full(u, "fp32")
+    assert (torch.linalg.norm(y32 - y64) / torch.linalg.norm(y64)).item() < 1e-5
+    op.close()

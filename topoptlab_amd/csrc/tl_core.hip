@@ -1,0 +1,962 @@
+// Problem handle, SIMP scaling, matrix-free K.u on structured Q4/H8 grids,
+// diagonal of K_ff and the device-resident preconditioned CG of the C ABI.
+//
+// Matrix-free operator (replaces the COO->CSC assembly and sparse product of
+// topoptlab/topology_optimization.py:419-451 and linear_solvers.py:114,130):
+//   y_n = sum_{elements e containing node n} s_e * (KE u_e)_{local(n,e)}
+// evaluated node-centrically, so every output DOF is written once (no
+// atomics, deterministic).  A workgroup owns a contiguous run of BS nodes of
+// one (x,y) node plane and walks a chunk of z-planes; each new z-plane (plus
+// its +-(ny+1) halo) is staged once in LDS and every thread keeps the 3x3x3
+// neighbourhood of its node in registers, rotating three plane slots so no
+// register moves are needed.  KE indices are compile-time constants with
+// wave-uniform addresses, so KE entries arrive as scalar operands.
+#include "tl_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace tl {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+int DevBuf::alloc(size_t nbytes) {
+  if (ptr && bytes >= nbytes) return TL_OK;
+  release();
+  if (nbytes == 0) return TL_OK;
+  TL_HIP(hipMalloc(&ptr, nbytes));
+  bytes = nbytes;
+  return TL_OK;
+}
+
+void DevBuf::release() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+constexpr int BS = 256;  // threads per workgroup of the vector/stencil kernels
+
+enum { FIN_STORE = 0, FIN_RHO = 1, FIN_PQ = 2, FIN_RR = 3, FIN_BB = 4, FIN_RR0 = 5 };
+
+__device__ inline void finalize(int fin, PcgState* st, double tot, double* out) {
+  switch (fin) {
+    case FIN_STORE:
+      *out = tot;
+      break;
+    case FIN_RHO:
+      st->rho = tot;
+      st->beta = (st->it == 0) ? 0.0 : tot / st->rho_prev;
+      break;
+    case FIN_PQ:
+      st->pq = tot;
+      st->alpha = st->rho / tot;
+      break;
+    case FIN_RR:
+      st->rr = tot;
+      st->rho_prev = st->rho;
+      st->it += 1;
+      st->done = (tot < st->tol2 || st->it >= st->maxiter) ? 1 : 0;
+      break;
+    case FIN_BB:
+      st->bb = tot;
+      break;
+    case FIN_RR0:
+      st->rr = tot;
+      st->it = 0;
+      st->done = (tot < st->tol2 || st->maxiter <= 0) ? 1 : 0;
+      break;
+  }
+}
+
+// ----------------------------------------------------------------------------
+// SIMP scale into the zero-bordered element array
+// ----------------------------------------------------------------------------
+__global__ void simp_pad_kernel(const double* __restrict__ x, double* __restrict__ s64,
+                                float* __restrict__ s32, int dim, int nelx, int nely,
+                                int nelz, double eps, double penal) {
+  const int64_t npad = int64_t(nelx + 2) * (nely + 2) * (dim == 3 ? nelz + 2 : 1);
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < npad;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    int64_t t = i;
+    const int py = int(t % (nely + 2)); t /= (nely + 2);
+    const int px = int(t % (nelx + 2)); t /= (nelx + 2);
+    const int pz = int(t);
+    const int ex = px - 1, ey = py - 1, ez = pz - 1;
+    double v = 0.0;
+    const bool in = ex >= 0 && ex < nelx && ey >= 0 && ey < nely &&
+                    (dim == 2 || (ez >= 0 && ez < nelz));
+    if (in) {
+      const int64_t e = (dim == 3 ? (int64_t(ez) * nelx + ex) * nely + ey
+                                  : int64_t(ex) * nely + ey);
+      v = eps + (1.0 - eps) * pow(x[e], penal);
+    }
+    s64[i] = v;
+    s32[i] = float(v);
+  }
+}
+
+// ----------------------------------------------------------------------------
+// fine operator kernels
+// ----------------------------------------------------------------------------
+struct KuArgs {
+  const void* u;        // input (T); with FUSEP: z
+  const void* pin;      // FUSEP: previous p (T)
+  void* pout;           // FUSEP: new p (T)
+  const double* beta;   // FUSEP: device scalar
+  const void* sp;       // padded scale (T)
+  const void* KE;       // element matrix (T), nd x nd row-major
+  const uint8_t* mask;  // per node
+  const void* b;        // RESID/JACOBI right-hand side (TB)
+  const void* dinv;     // JACOBI (T)
+  double omega;
+  void* y;              // output (TO)
+  double* partials;
+  unsigned* counter;
+  double* dot_out;
+  PcgState* st;
+  int fin;
+  const int* done;
+  int nelx, nely, nelz, nx, ny, nz, plane, zchunk;
+};
+
+template <typename T>
+__device__ inline T ldval(const void* p, int64_t i) {
+  return static_cast<const T*>(p)[i];
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO>
+__device__ inline void ku_epilogue(const KuArgs& a, int64_t node, const T (&uc)[NDPN],
+                                   const T (&acc)[NDPN], double& dot) {
+  const uint8_t m = a.mask[node];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) {
+    const int64_t g = node * NDPN + d;
+    const bool fixed = (m >> d) & 1;
+    if constexpr (MODE == KU_STORE || MODE == KU_STORE_DOT) {
+      const T v = fixed ? T(0) : acc[d];
+      static_cast<TO*>(a.y)[g] = TO(v);
+      if constexpr (MODE == KU_STORE_DOT) dot += double(uc[d]) * double(v);
+    } else if constexpr (MODE == KU_RESID) {
+      const T v = fixed ? T(0) : T(ldval<TB>(a.b, g)) - acc[d];
+      static_cast<TO*>(a.y)[g] = TO(v);
+    } else {
+      const T bb = T(ldval<TB>(a.b, g));
+      const T v = fixed ? T(0)
+                        : uc[d] + T(a.omega) * ldval<T>(a.dinv, g) * (bb - acc[d]);
+      static_cast<TO*>(a.y)[g] = TO(v);
+      if constexpr (MODE == KU_JACOBI_DOT) dot += double(bb) * double(v);
+    }
+  }
+}
+
+template <typename T>
+__device__ inline T stage_val(const KuArgs& a, int64_t g, bool fusep) {
+  if (fusep) return ldval<T>(a.u, g) + T(*a.beta) * ldval<T>(a.pin, g);
+  return ldval<T>(a.u, g);
+}
+
+// Contribution of the 4 (2D) / 8 (3D) elements around a node.
+// U: neighbour values [3 slots][9][NDPN]; slot index of relative plane rz.
+template <int NDPN, typename T, int SM, int SC, int SP, bool D3>
+__device__ inline void node_sum(const KuArgs& a, const T (&U)[3][9][NDPN],
+                                const T (&S)[3][4], int sdn, int sup, T (&acc)[NDPN]) {
+  constexpr int ND = (D3 ? 8 : 4) * NDPN;
+  // An opaque zero offset keeps the KE loads inside the z loop: hoisted out
+  // of it, the 576 loop-invariant entries would be pinned in vector registers
+  // and spill.  Inside the loop they are wave-uniform scalar loads.
+  // KE is read through the constant address space: the buffer is never
+  // written while a kernel runs, so wave-uniform reads become scalar loads
+  // whose results feed the FMAs as SGPR operands.
+  int opq;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(opq));
+  const __attribute__((address_space(4))) T* KE =
+      (const __attribute__((address_space(4))) T*)(static_cast<const T*>(a.KE)) + opq;
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
+#pragma unroll
+  for (int oz = (D3 ? -1 : 0); oz <= 0; ++oz) {
+#pragma unroll
+    for (int ox = -1; ox <= 0; ++ox) {
+#pragma unroll
+      for (int oy = -1; oy <= 0; ++oy) {
+        const T s = (oz < 0) ? S[sdn][(ox + 1) * 2 + (oy + 1)] : S[sup][(ox + 1) * 2 + (oy + 1)];
+        const int la = loc_of(-ox, -oy, -oz);
+        T t[NDPN];
+#pragma unroll
+        for (int d = 0; d < NDPN; ++d) t[d] = T(0);
+#pragma unroll
+        for (int lb = 0; lb < (D3 ? 8 : 4); ++lb) {
+          const int rx = ox + loc_dx(lb), ry = oy + loc_dy(lb), rz = oz + loc_dz(lb);
+          const int slot = (rz < 0) ? SM : (rz == 0 ? SC : SP);
+          const int nb = (rx + 1) * 3 + (ry + 1);
+#pragma unroll
+          for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+            for (int c = 0; c < NDPN; ++c)
+              t[d] += KE[(la * NDPN + d) * ND + lb * NDPN + c] * U[slot][nb][c];
+        }
+#pragma unroll
+        for (int d = 0; d < NDPN; ++d) acc[d] += s * t[d];
+      }
+    }
+  }
+}
+
+template <int NDPN, typename T>
+__device__ inline void stage_plane(const KuArgs& a, T* lds, int pz, int64_t j0, bool fusep) {
+  const int L = BS + 2 * a.ny + 2;
+  const int64_t base = j0 - a.ny - 1;
+  const bool zin = pz >= 0 && pz < a.nz;
+  for (int t = threadIdx.x; t < L * NDPN; t += BS) {
+    const int64_t jj = base + t / NDPN;
+    T v = T(0);
+    if (zin && jj >= 0 && jj < a.plane)
+      v = stage_val<T>(a, (int64_t(pz) * a.plane + jj) * NDPN + (t % NDPN), fusep);
+    lds[t] = v;
+  }
+}
+
+template <int NDPN, typename T, int SLOT>
+__device__ inline void read_nbrs(const KuArgs& a, const T* lds, T (&U)[3][9][NDPN]) {
+  const int c0 = threadIdx.x + a.ny + 1;
+#pragma unroll
+  for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int c = 0; c < NDPN; ++c)
+        U[SLOT][(dx + 1) * 3 + (dy + 1)][c] = lds[(c0 + dx * a.ny + dy) * NDPN + c];
+}
+
+template <typename T>
+__device__ inline void load_dens(const KuArgs& a, int ix, int iy, int ez, T (&S)[4]) {
+  const T* sp = static_cast<const T*>(a.sp);
+  const int64_t rowz = int64_t(ez + 1) * (a.nelx + 2);
+#pragma unroll
+  for (int ox = -1; ox <= 0; ++ox)
+#pragma unroll
+    for (int oy = -1; oy <= 0; ++oy)
+      S[(ox + 1) * 2 + (oy + 1)] = sp[(rowz + ix + ox + 1) * (a.nely + 2) + iy + oy + 1];
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int SA,
+          int SB, int SC>
+__device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t j0,
+                              int64_t j, int ix, int iy, bool active, T (&U)[3][9][NDPN],
+                              T (&S)[3][4], double& dot) {
+  // plane iz+1 -> slot SC (LDS buffer by parity), element layer iz -> S[SB]
+  T* buf = ((iz + 1) & 1) ? lds1 : lds0;
+  stage_plane<NDPN, T>(a, buf, iz + 1, j0, FUSEP);
+  load_dens<T>(a, ix, iy, iz, S[SB]);
+  __syncthreads();
+  read_nbrs<NDPN, T, SC>(a, buf, U);
+  T acc[NDPN];
+  node_sum<NDPN, T, SA, SB, SC, true>(a, U, S, SA, SB, acc);
+  if (active) {
+    const int64_t node = int64_t(iz) * a.plane + j;
+    T uc[NDPN];
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d) uc[d] = U[SB][4][d];
+    if constexpr (FUSEP) {
+#pragma unroll
+      for (int d = 0; d < NDPN; ++d) static_cast<T*>(a.pout)[node * NDPN + d] = uc[d];
+    }
+    ku_epilogue<NDPN, T, MODE, TB, TO>(a, node, uc, acc, dot);
+  }
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP>
+__global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
+  if (a.done && *a.done) return;
+  extern __shared__ double lds_raw[];
+  const int L = BS + 2 * a.ny + 2;
+  T* lds0 = reinterpret_cast<T*>(lds_raw);
+  T* lds1 = lds0 + L * NDPN;
+  const int64_t j0 = int64_t(blockIdx.x) * BS;
+  const int64_t j = j0 + threadIdx.x;
+  const bool inplane = j < a.plane;
+  const int64_t jc = inplane ? j : a.plane - 1;
+  const int ix = int(jc / a.ny), iy = int(jc % a.ny);
+  const int z0 = blockIdx.y * a.zchunk;
+  const int z1 = min(a.nz, z0 + a.zchunk);
+  T U[3][9][NDPN];
+  T S[3][4];
+  double dot = 0.0;
+  // prologue: planes z0-1 (slot 0) and z0 (slot 1), element layer z0-1 (S[0])
+  stage_plane<NDPN, T>(a, lds0, z0 - 1, j0, FUSEP);
+  stage_plane<NDPN, T>(a, lds1, z0, j0, FUSEP);
+  load_dens<T>(a, ix, iy, z0 - 1, S[0]);
+  __syncthreads();
+  read_nbrs<NDPN, T, 0>(a, lds0, U);
+  read_nbrs<NDPN, T, 1>(a, lds1, U);
+  __syncthreads();
+  // planes of node iz use LDS buffer (iz & 1); rotating slots (0,1,2)
+  for (int iz = z0; iz < z1; iz += 3) {
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, 0, 1, 2>(a, lds0, lds1, iz, j0, j, ix, iy, inplane, U, S, dot);
+    __syncthreads();
+    if (iz + 1 >= z1) break;
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, 1, 2, 0>(a, lds0, lds1, iz + 1, j0, j, ix, iy, inplane, U, S, dot);
+    __syncthreads();
+    if (iz + 2 >= z1) break;
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, 2, 0, 1>(a, lds0, lds1, iz + 2, j0, j, ix, iy, inplane, U, S, dot);
+    __syncthreads();
+  }
+  if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
+    __shared__ double sh[BS / 64 + 2];
+    double v[1] = {dot};
+    block_sum<1>(v, sh);
+    double tot[1];
+    if (last_block_total<1>(v, a.partials, a.counter, tot, sh) && threadIdx.x == 0)
+      finalize(a.fin, a.st, tot[0], a.dot_out);
+  }
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP>
+__global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
+  if (a.done && *a.done) return;
+  extern __shared__ double lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);
+  const int64_t j0 = int64_t(blockIdx.x) * BS;
+  const int64_t j = j0 + threadIdx.x;
+  const bool active = j < a.plane;
+  const int64_t jc = active ? j : a.plane - 1;
+  const int ix = int(jc / a.ny), iy = int(jc % a.ny);
+  T U[3][9][NDPN];
+  T S[3][4];
+  stage_plane<NDPN, T>(a, lds, 0, j0, FUSEP);
+  // 2D padded array: rows of (nely+2), one layer -> use ez = -1 addressing
+  {
+    const T* sp = static_cast<const T*>(a.sp);
+#pragma unroll
+    for (int ox = -1; ox <= 0; ++ox)
+#pragma unroll
+      for (int oy = -1; oy <= 0; ++oy)
+        S[1][(ox + 1) * 2 + (oy + 1)] = sp[int64_t(ix + ox + 1) * (a.nely + 2) + iy + oy + 1];
+  }
+  __syncthreads();
+  read_nbrs<NDPN, T, 1>(a, lds, U);
+  T acc[NDPN];
+  node_sum<NDPN, T, 0, 1, 2, false>(a, U, S, 0, 1, acc);
+  double dot = 0.0;
+  if (active) {
+    T uc[NDPN];
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d) uc[d] = U[1][4][d];
+    if constexpr (FUSEP) {
+#pragma unroll
+      for (int d = 0; d < NDPN; ++d) static_cast<T*>(a.pout)[j * NDPN + d] = uc[d];
+    }
+    ku_epilogue<NDPN, T, MODE, TB, TO>(a, j, uc, acc, dot);
+  }
+  if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
+    __shared__ double sh[BS / 64 + 2];
+    double v[1] = {dot};
+    block_sum<1>(v, sh);
+    double tot[1];
+    if (last_block_total<1>(v, a.partials, a.counter, tot, sh) && threadIdx.x == 0)
+      finalize(a.fin, a.st, tot[0], a.dot_out);
+  }
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP>
+static int launch_ku(const tl_problem* P, KuArgs a) {
+  const Grid& g = P->g;
+  a.nelx = g.nelx; a.nely = g.nely; a.nelz = g.nelz;
+  a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.plane = g.nx * g.ny;
+  const int L = BS + 2 * g.ny + 2;
+  const unsigned nbx = unsigned((int64_t(a.plane) + BS - 1) / BS);
+  if (g.dim == 3) {
+    // z chunk: enough workgroups to fill 256 CUs, each chunk >= 4 planes
+    int zc = g.nz;
+    while (zc > 4 && int64_t(nbx) * ((g.nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+    a.zchunk = zc;
+    dim3 grid(nbx, unsigned((g.nz + zc - 1) / zc));
+    if (int64_t(grid.x) * grid.y > kMaxRedBlocks && (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)) {
+      set_error("fine operator grid exceeds the reduction slot count");
+      return TL_EINVAL;
+    }
+    hipLaunchKernelGGL((ku3d_kernel<NDPN, T, MODE, TB, TO, FUSEP>), grid, dim3(BS),
+                       size_t(2) * L * NDPN * sizeof(T), P->stream, a);
+  } else {
+    a.zchunk = 1;
+    if (nbx > unsigned(kMaxRedBlocks) && (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)) {
+      set_error("fine operator grid exceeds the reduction slot count");
+      return TL_EINVAL;
+    }
+    hipLaunchKernelGGL((ku2d_kernel<NDPN, T, MODE, TB, TO, FUSEP>), dim3(nbx), dim3(BS),
+                       size_t(L) * NDPN * sizeof(T), P->stream, a);
+  }
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+template <int NDPN, typename T>
+static int dispatch_mode(const tl_problem* P, const KuArgs& a, int mode, bool fusep, bool b64,
+                         bool out64) {
+  // (mode, fusep, b type, out type) combinations used by the solver
+  if (mode == KU_STORE) {
+    if (out64) return launch_ku<NDPN, T, KU_STORE, double, double, false>(P, a);
+    return launch_ku<NDPN, T, KU_STORE, T, T, false>(P, a);
+  }
+  if (mode == KU_STORE_DOT) {
+    if (fusep) return launch_ku<NDPN, T, KU_STORE_DOT, T, T, true>(P, a);
+    return launch_ku<NDPN, T, KU_STORE_DOT, T, T, false>(P, a);
+  }
+  if (mode == KU_RESID) {
+    if (b64) return launch_ku<NDPN, T, KU_RESID, double, T, false>(P, a);
+    return launch_ku<NDPN, T, KU_RESID, T, T, false>(P, a);
+  }
+  if (mode == KU_JACOBI) {
+    if (b64) return launch_ku<NDPN, T, KU_JACOBI, double, T, false>(P, a);
+    return launch_ku<NDPN, T, KU_JACOBI, T, T, false>(P, a);
+  }
+  if (mode == KU_JACOBI_DOT) {
+    if (b64) return launch_ku<NDPN, T, KU_JACOBI_DOT, double, double, false>(P, a);
+  }
+  set_error("unsupported operator mode");
+  return TL_EINVAL;
+}
+
+static int fine_apply_dispatch(tl_problem* P, KuArgs& a, int prec, int mode, bool fusep, bool b64,
+                               bool out64);
+
+// Generic entry used by the solver and the V-cycle.  prec selects T; b is
+// fp64 when b64; the output type is T except for KU_JACOBI_DOT (fp64) and
+// KU_STORE with prec fp32 and out64 (fp64 output).
+int fine_apply_ex(tl_problem* P, const void* u, void* y, int prec, int mode, const void* b,
+                  bool b64, const void* dinv, double omega, const void* zfuse_pin,
+                  void* pout, const double* beta, double* dot_out, int fin, const int* done,
+                  bool out64) {
+  KuArgs a{};
+  a.u = u; a.y = y; a.b = b; a.dinv = dinv; a.omega = omega;
+  a.pin = zfuse_pin; a.pout = pout; a.beta = beta;
+  a.mask = P->op.mask.as<uint8_t>();
+  a.partials = P->partials.as<double>();
+  a.counter = P->counter.as<unsigned>();
+  a.dot_out = dot_out;
+  a.st = P->state.as<PcgState>();
+  a.fin = fin;
+  a.done = done;
+  const bool fusep = zfuse_pin != nullptr;
+  ProfEvent ev{};
+  const bool prof = P->prof_on;
+  if (prof) {
+    if (P->prof_free.empty()) {
+      TL_HIP(hipEventCreate(&ev.start));
+      TL_HIP(hipEventCreate(&ev.stop));
+    } else {
+      ev = P->prof_free.back();
+      P->prof_free.pop_back();
+    }
+    // kind 0: fp64 products of the PCG loop; kind 1: operator inside the V-cycle
+    ev.kind = (mode == KU_STORE || mode == KU_STORE_DOT) ? 0 : 1;
+    TL_HIP(hipEventRecord(ev.start, P->stream));
+  }
+  const int rc = fine_apply_dispatch(P, a, prec, mode, fusep, b64, out64);
+  if (prof) {
+    TL_HIP(hipEventRecord(ev.stop, P->stream));
+    P->prof_used.push_back(ev);
+  }
+  return rc;
+}
+
+static int fine_apply_dispatch(tl_problem* P, KuArgs& a, int prec, int mode, bool fusep, bool b64,
+                               bool out64) {
+  if (prec == TL_PREC_FP64) {
+    a.sp = P->op.sp64.ptr; a.KE = P->op.KE64.ptr;
+    switch (P->ndpn) {
+      case 1: return dispatch_mode<1, double>(P, a, mode, fusep, b64, out64);
+      case 2: return dispatch_mode<2, double>(P, a, mode, fusep, b64, out64);
+      case 3: return dispatch_mode<3, double>(P, a, mode, fusep, b64, out64);
+    }
+  } else {
+    a.sp = P->op.sp32.ptr; a.KE = P->op.KE32.ptr;
+    switch (P->ndpn) {
+      case 1: return dispatch_mode<1, float>(P, a, mode, fusep, b64, out64);
+      case 2: return dispatch_mode<2, float>(P, a, mode, fusep, b64, out64);
+      case 3: return dispatch_mode<3, float>(P, a, mode, fusep, b64, out64);
+    }
+  }
+  set_error("unsupported DOFs per node");
+  return TL_EINVAL;
+}
+
+// ----------------------------------------------------------------------------
+// diagonal of K_ff
+// ----------------------------------------------------------------------------
+template <int NDPN>
+__global__ void diag_kernel(const double* __restrict__ sp, const double* __restrict__ KE,
+                            const uint8_t* __restrict__ mask, double* __restrict__ d64,
+                            float* __restrict__ dinv32, double* __restrict__ dinv64, int dim,
+                            int nelx, int nely, int nelz, int ny, int64_t nnodes) {
+  const int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (n >= nnodes) return;
+  const int nd = (dim == 3 ? 8 : 4) * NDPN;
+  const int plane = (nelx + 1) * ny;
+  const int iz = int(n / plane);
+  const int64_t j = n % plane;
+  const int ix = int(j / ny), iy = int(j % ny);
+  double acc[NDPN];
+  for (int d = 0; d < NDPN; ++d) acc[d] = 0.0;
+  for (int oz = (dim == 3 ? -1 : 0); oz <= 0; ++oz)
+    for (int ox = -1; ox <= 0; ++ox)
+      for (int oy = -1; oy <= 0; ++oy) {
+        const int64_t pidx =
+            (dim == 3 ? (int64_t(iz + oz + 1) * (nelx + 2) + ix + ox + 1) * (nely + 2)
+                      : int64_t(ix + ox + 1) * (nely + 2)) + iy + oy + 1;
+        const double s = sp[pidx];
+        const int la = loc_of(-ox, -oy, -oz);
+        for (int d = 0; d < NDPN; ++d) acc[d] += s * KE[(la * NDPN + d) * nd + la * NDPN + d];
+      }
+  const uint8_t m = mask[n];
+  for (int d = 0; d < NDPN; ++d) {
+    const bool fixed = (m >> d) & 1;
+    const int64_t g = n * NDPN + d;
+    if (d64) d64[g] = fixed ? 0.0 : acc[d];
+    const double inv = (fixed || acc[d] == 0.0) ? 0.0 : 1.0 / acc[d];
+    if (dinv64) dinv64[g] = inv;
+    if (dinv32) dinv32[g] = float(inv);
+  }
+}
+
+int fine_diag_to(tl_problem* P, double* d64, double* dinv64, float* dinv32) {
+  const Grid& g = P->g;
+  const unsigned nb = unsigned((g.nnodes + BS - 1) / BS);
+#define TL_DIAG(N)                                                                       \
+  hipLaunchKernelGGL(diag_kernel<N>, dim3(nb), dim3(BS), 0, P->stream,                  \
+                     P->op.sp64.as<double>(), P->op.KE64.as<double>(),                  \
+                     P->op.mask.as<uint8_t>(), d64, dinv32, dinv64, g.dim, g.nelx, g.nely, \
+                     g.nelz, g.ny, g.nnodes)
+  switch (P->ndpn) {
+    case 1: TL_DIAG(1); break;
+    case 2: TL_DIAG(2); break;
+    case 3: TL_DIAG(3); break;
+    default: set_error("unsupported DOFs per node"); return TL_EINVAL;
+  }
+#undef TL_DIAG
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+int fine_diag(tl_problem* P) {
+  return fine_diag_to(P, nullptr, P->op.dinv64.as<double>(), P->op.dinv32.as<float>());
+}
+
+// ----------------------------------------------------------------------------
+// vector kernels of the PCG loop
+// ----------------------------------------------------------------------------
+template <int NV>
+__device__ inline void reduce_finish(double (&v)[NV], double* partials, unsigned* counter,
+                                     int fin, PcgState* st, double* out) {
+  __shared__ double sh[NV * (BS / 64) + 4];
+  block_sum<NV>(v, sh);
+  double tot[NV];
+  if (last_block_total<NV>(v, partials, counter, tot, sh) && threadIdx.x == 0)
+    finalize(fin, st, tot[0], out);
+}
+
+static unsigned red_blocks(int64_t n) {
+  int64_t nb = (n + BS - 1) / BS;
+  nb = std::min<int64_t>(nb, 1024);
+  return unsigned(std::max<int64_t>(nb, 1));
+}
+
+// out = a.b (fin selects where the total goes)
+__global__ __launch_bounds__(BS) void dot_kernel(const double* __restrict__ a,
+                                                 const double* __restrict__ b, int64_t n,
+                                                 double* partials, unsigned* counter, int fin,
+                                                 PcgState* st, double* out, const int* done) {
+  if (done && *done) return;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS)
+    s += a[i] * b[i];
+  double v[1] = {s};
+  reduce_finish<1>(v, partials, counter, fin, st, out);
+}
+
+// r = b - y (masked by y's zeros already); rr
+__global__ __launch_bounds__(BS) void resid_kernel(const double* __restrict__ b,
+                                                   const double* __restrict__ y,
+                                                   const uint8_t* __restrict__ mask, int ndpn,
+                                                   double* __restrict__ r, int64_t n,
+                                                   double* partials, unsigned* counter, int fin,
+                                                   PcgState* st, double* out) {
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
+    const bool fixed = (mask[i / ndpn] >> (i % ndpn)) & 1;
+    const double v = fixed ? 0.0 : b[i] - y[i];
+    if (r) r[i] = v;
+    s += v * v;
+  }
+  double v[1] = {s};
+  reduce_finish<1>(v, partials, counter, fin, st, out);
+}
+
+// x += alpha p ; r -= alpha q ; rr
+__global__ __launch_bounds__(BS) void update_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ q, int64_t n,
+                                                    double* partials, unsigned* counter,
+                                                    PcgState* st, const int* done) {
+  if (*done) return;
+  const double alpha = st->alpha;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
+    x[i] += alpha * p[i];
+    const double rv = r[i] - alpha * q[i];
+    r[i] = rv;
+    s += rv * rv;
+  }
+  double v[1] = {s};
+  reduce_finish<1>(v, partials, counter, FIN_RR, st, nullptr);
+}
+
+// z = dinv * r (or z = r when dinv == nullptr); rho = r.z
+__global__ __launch_bounds__(BS) void jacobi_prec_kernel(const double* __restrict__ r,
+                                                         const double* __restrict__ dinv,
+                                                         double* __restrict__ z, int64_t n,
+                                                         double* partials, unsigned* counter,
+                                                         PcgState* st, const int* done) {
+  if (*done) return;
+  double s = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
+    const double zv = dinv ? dinv[i] * r[i] : r[i];
+    z[i] = zv;
+    s += r[i] * zv;
+  }
+  double v[1] = {s};
+  reduce_finish<1>(v, partials, counter, FIN_RHO, st, nullptr);
+}
+
+// masked copy (fixed DOFs -> 0)
+__global__ void masked_copy_kernel(const double* __restrict__ src, double* __restrict__ dst,
+                                   const uint8_t* __restrict__ mask, int ndpn, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const bool fixed = (mask[i / ndpn] >> (i % ndpn)) & 1;
+    dst[i] = fixed ? 0.0 : src[i];
+  }
+}
+
+__global__ void cast_kernel_d2f(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = float(a[i]);
+}
+
+__global__ void cast_kernel_f2d(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = double(a[i]);
+}
+
+__global__ void init_state_kernel(PcgState* st, double rtol, double atol, int maxiter) {
+  // tolerance as linear_solvers.py:107-109 (scipy _get_atol_rtol): max(atol, rtol*||b||)
+  const double tol = fmax(atol, rtol * sqrt(st->bb));
+  st->tol2 = tol * tol;
+  st->maxiter = maxiter;
+  st->it = 0;
+  st->done = 0;
+  st->rho = st->rho_prev = st->pq = st->alpha = st->beta = 0.0;
+}
+
+static unsigned grid_for(int64_t n) {
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>((n + BS - 1) / BS, 8192)));
+}
+
+int reduce_dot(tl_problem* P, const double* a, const double* b, int64_t n, double* out_dev) {
+  hipLaunchKernelGGL(dot_kernel, dim3(red_blocks(n)), dim3(BS), 0, P->stream, a, b, n,
+                     P->partials.as<double>(), P->counter.as<unsigned>(), FIN_STORE,
+                     P->state.as<PcgState>(), out_dev, nullptr);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+}  // namespace tl
+
+// ----------------------------------------------------------------------------
+// handle lifetime
+// ----------------------------------------------------------------------------
+tl_problem::~tl_problem() {
+  for (auto& ev : prof_used) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
+  for (auto& ev : prof_free) { (void)hipEventDestroy(ev.start); (void)hipEventDestroy(ev.stop); }
+  op.KE64.release(); op.KE32.release(); op.KE16.release();
+  op.sp64.release(); op.sp32.release();
+  op.dinv64.release(); op.dinv32.release(); op.mask.release();
+  for (auto& l : mg.lv) {
+    l.mask.release(); l.st64.release(); l.st32.release(); l.dinv64.release();
+    l.dinv32.release(); l.em.release(); l.vb.release(); l.vx.release(); l.vr.release();
+    l.vt.release();
+  }
+  mg.G1.release(); mg.W.release(); mg.cinv.release();
+  r.release(); z.release(); p.release(); p2.release(); q.release(); t.release();
+  partials.release(); counter.release(); state.release();
+}
+
+using namespace tl;
+
+extern "C" {
+
+const char* tl_last_error(void) { return tl::g_last_error.c_str(); }
+
+int64_t tl_ndof(const tl_problem* p) { return p ? p->ndof : -1; }
+int64_t tl_nel(const tl_problem* p) { return p ? p->g.nel : -1; }
+
+int tl_create(const tl_grid_desc* d, tl_problem** out) {
+  TL_ARG(d && out, "null argument");
+  TL_ARG(d->ndim == 2 || d->ndim == 3, "ndim must be 2 or 3");
+  TL_ARG(d->nelx > 0 && d->nely > 0 && (d->ndim == 2 || d->nelz > 0), "element counts must be positive");
+  TL_ARG(d->ndpn == 1 || d->ndpn == d->ndim, "ndpn must be 1 or ndim");
+  TL_ARG(d->KE != nullptr, "KE is null");
+  TL_ARG(d->nfixed >= 0 && (d->nfixed == 0 || d->fixed), "fixed DOF list is null");
+  const int64_t nmax = int64_t(1) << 31;
+  const int64_t nodes = int64_t(d->nelx + 1) * (d->nely + 1) * (d->ndim == 3 ? d->nelz + 1 : 1);
+  TL_ARG(nodes * d->ndpn < nmax, "grid too large for 32-bit plane indexing");
+  auto* P = new tl_problem();
+  P->g.set(d->ndim, d->nelx, d->nely, d->nelz);
+  P->ndpn = d->ndpn;
+  P->nd = (d->ndim == 3 ? 8 : 4) * d->ndpn;
+  P->ndof = P->g.nnodes * d->ndpn;
+  P->eps = d->eps;
+  P->penal = d->penal;
+  P->stream = static_cast<hipStream_t>(d->stream);
+  const int nd = P->nd;
+  P->KEh.assign(d->KE, d->KE + nd * nd);
+  std::vector<uint8_t> mask(P->g.nnodes, 0);
+  for (int64_t i = 0; i < d->nfixed; ++i) {
+    const int64_t f = d->fixed[i];
+    if (f < 0 || f >= P->ndof) {
+      delete P;
+      set_error("fixed DOF index out of range");
+      return TL_EINVAL;
+    }
+    mask[f / d->ndpn] |= uint8_t(1u << (f % d->ndpn));
+  }
+  P->fixed_h.assign(d->fixed, d->fixed + d->nfixed);
+  std::vector<float> ke32(nd * nd);
+  for (int i = 0; i < nd * nd; ++i) ke32[i] = float(P->KEh[i]);
+  auto fail = [&](int rc) { delete P; return rc; };
+  int rc;
+  if ((rc = P->op.KE64.alloc(sizeof(double) * nd * nd)) ||
+      (rc = P->op.KE32.alloc(sizeof(float) * nd * nd)) ||
+      (rc = P->op.mask.alloc(P->g.nnodes)) ||
+      (rc = P->op.sp64.alloc(sizeof(double) * P->g.npad())) ||
+      (rc = P->op.sp32.alloc(sizeof(float) * P->g.npad())) ||
+      (rc = P->op.dinv64.alloc(sizeof(double) * P->ndof)) ||
+      (rc = P->op.dinv32.alloc(sizeof(float) * P->ndof)) ||
+      (rc = P->partials.alloc(sizeof(double) * 4 * kMaxRedBlocks)) ||
+      (rc = P->counter.alloc(sizeof(unsigned) * 4)) ||
+      (rc = P->state.alloc(sizeof(PcgState))))
+    return fail(rc);
+  hipError_t e;
+  if ((e = hipMemcpy(P->op.KE64.ptr, P->KEh.data(), sizeof(double) * nd * nd, hipMemcpyHostToDevice)) ||
+      (e = hipMemcpy(P->op.KE32.ptr, ke32.data(), sizeof(float) * nd * nd, hipMemcpyHostToDevice)) ||
+      (e = hipMemcpy(P->op.mask.ptr, mask.data(), P->g.nnodes, hipMemcpyHostToDevice)) ||
+      (e = hipMemset(P->counter.ptr, 0, sizeof(unsigned) * 4)) ||
+      (e = hipMemset(P->state.ptr, 0, sizeof(PcgState))) ||
+      (e = hipMemset(P->op.sp64.ptr, 0, sizeof(double) * P->g.npad())) ||
+      (e = hipMemset(P->op.sp32.ptr, 0, sizeof(float) * P->g.npad()))) {
+    set_error(std::string("tl_create: ") + hipGetErrorString(e));
+    return fail(TL_EHIP);
+  }
+  *out = P;
+  return TL_OK;
+}
+
+void tl_destroy(tl_problem* p) { delete p; }
+
+int tl_set_stream(tl_problem* p, void* stream) {
+  TL_ARG(p, "null handle");
+  p->stream = static_cast<hipStream_t>(stream);
+  return TL_OK;
+}
+
+int tl_set_density(tl_problem* P, const double* xPhys) {
+  TL_ARG(P && xPhys, "null argument");
+  const Grid& g = P->g;
+  hipLaunchKernelGGL(simp_pad_kernel, dim3(grid_for(g.npad())), dim3(BS), 0, P->stream, xPhys,
+                     P->op.sp64.as<double>(), P->op.sp32.as<float>(), g.dim, g.nelx, g.nely,
+                     g.nelz, P->eps, P->penal);
+  TL_CHECK_LAUNCH();
+  TL_TRY(fine_diag(P));
+  P->op.have_density = true;
+  P->mg.ready = false;
+  return TL_OK;
+}
+
+int tl_apply_K(tl_problem* P, const double* u, double* y, int32_t prec) {
+  TL_ARG(P && u && y, "null argument");
+  TL_ARG(P->op.have_density, "tl_set_density must be called first");
+  TL_ARG(prec == TL_PREC_FP64 || prec == TL_PREC_FP32, "prec must be FP64 or FP32");
+  TL_TRY(P->t.alloc(sizeof(double) * P->ndof));
+  const unsigned nb = grid_for(P->ndof);
+  hipLaunchKernelGGL(masked_copy_kernel, dim3(nb), dim3(BS), 0, P->stream, u, P->t.as<double>(),
+                     P->op.mask.as<uint8_t>(), P->ndpn, P->ndof);
+  TL_CHECK_LAUNCH();
+  if (prec == TL_PREC_FP64)
+    return fine_apply_ex(P, P->t.ptr, y, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0,
+                         nullptr, nullptr, nullptr, nullptr, FIN_STORE, nullptr, true);
+  // fp32: cast input, apply, widen output
+  TL_TRY(P->z.alloc(sizeof(double) * P->ndof));
+  float* u32 = reinterpret_cast<float*>(P->z.ptr);
+  float* y32 = u32 + P->ndof;  // z holds 2*ndof floats
+  hipLaunchKernelGGL(cast_kernel_d2f, dim3(nb), dim3(BS), 0, P->stream, P->t.as<double>(), u32,
+                     P->ndof);
+  TL_CHECK_LAUNCH();
+  TL_TRY(fine_apply_ex(P, u32, y32, TL_PREC_FP32, KU_STORE, nullptr, false, nullptr, 0.0,
+                       nullptr, nullptr, nullptr, nullptr, FIN_STORE, nullptr, false));
+  hipLaunchKernelGGL(cast_kernel_f2d, dim3(nb), dim3(BS), 0, P->stream, y32, y, P->ndof);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+int tl_profile(tl_problem* P, int32_t enable) {
+  TL_ARG(P, "null handle");
+  P->prof_on = enable != 0;
+  return TL_OK;
+}
+
+int tl_profile_read(tl_problem* P, int32_t kind, double* total_ms, int64_t* launches) {
+  TL_ARG(P && total_ms && launches, "null argument");
+  double tot = 0.0;
+  int64_t cnt = 0;
+  std::vector<ProfEvent> keep;
+  for (auto& ev : P->prof_used) {
+    if (ev.kind != kind) {
+      keep.push_back(ev);
+      continue;
+    }
+    TL_HIP(hipEventSynchronize(ev.stop));
+    float ms = 0.f;
+    TL_HIP(hipEventElapsedTime(&ms, ev.start, ev.stop));
+    tot += ms;
+    ++cnt;
+    P->prof_free.push_back(ev);
+  }
+  P->prof_used.swap(keep);
+  *total_ms = tot;
+  *launches = cnt;
+  return TL_OK;
+}
+
+int tl_diag_K(tl_problem* P, double* d) {
+  TL_ARG(P && d, "null argument");
+  TL_ARG(P->op.have_density, "tl_set_density must be called first");
+  return fine_diag_to(P, d, nullptr, nullptr);
+}
+
+int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
+             tl_solve_info* info) {
+  TL_ARG(P && f && u && o, "null argument");
+  TL_ARG(P->op.have_density, "tl_set_density must be called first");
+  TL_ARG(o->precond >= TL_PRECOND_NONE && o->precond <= TL_PRECOND_GMG, "unknown preconditioner");
+  TL_ARG(o->precond != TL_PRECOND_GMG || P->mg.ready, "tl_setup_mg must be called first");
+  TL_ARG(o->rtol >= 0.0 && o->atol >= 0.0, "tolerances must be non-negative");
+  const int64_t n = P->ndof;
+  TL_TRY(P->r.alloc(sizeof(double) * n));
+  TL_TRY(P->z.alloc(sizeof(double) * n));
+  TL_TRY(P->p.alloc(sizeof(double) * n));
+  TL_TRY(P->p2.alloc(sizeof(double) * n));
+  TL_TRY(P->q.alloc(sizeof(double) * n));
+  TL_TRY(P->t.alloc(sizeof(double) * n));
+  double* r = P->r.as<double>();
+  double* z = P->z.as<double>();
+  double* p = P->p.as<double>();
+  double* q = P->q.as<double>();
+  double* t = P->t.as<double>();
+  PcgState* st = P->state.as<PcgState>();
+  const int* done = &st->done;
+  double* partials = P->partials.as<double>();
+  unsigned* counter = P->counter.as<unsigned>();
+  const uint8_t* mask = P->op.mask.as<uint8_t>();
+  const unsigned nb = grid_for(n), rb = red_blocks(n);
+  hipStream_t s = P->stream;
+  // b (masked) -> t ; bb
+  hipLaunchKernelGGL(masked_copy_kernel, dim3(nb), dim3(BS), 0, s, f, t, mask, P->ndpn, n);
+  TL_CHECK_LAUNCH();
+  hipLaunchKernelGGL(dot_kernel, dim3(rb), dim3(BS), 0, s, t, t, n, partials, counter, FIN_BB,
+                     st, nullptr, nullptr);
+  TL_CHECK_LAUNCH();
+  hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(1), 0, s, st, o->rtol, o->atol, o->maxiter);
+  TL_CHECK_LAUNCH();
+  // x0: masked copy of u, or zero
+  if (o->x0_given) {
+    hipLaunchKernelGGL(masked_copy_kernel, dim3(nb), dim3(BS), 0, s, u, u, mask, P->ndpn, n);
+    TL_CHECK_LAUNCH();
+    TL_TRY(fine_apply_ex(P, u, q, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0, nullptr,
+                         nullptr, nullptr, nullptr, FIN_STORE, nullptr, true));
+    hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn, r, n,
+                       partials, counter, FIN_RR0, st, nullptr);
+  } else {
+    TL_HIP(hipMemsetAsync(u, 0, sizeof(double) * n, s));
+    TL_HIP(hipMemsetAsync(q, 0, sizeof(double) * n, s));
+    hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn, r, n,
+                       partials, counter, FIN_RR0, st, nullptr);
+  }
+  TL_CHECK_LAUNCH();
+  // search directions ping-pong between two buffers: the fused product
+  // reads p_k (halo included) while it writes p_{k+1}
+  double* pp[2] = {p, P->p2.as<double>()};
+  TL_HIP(hipMemsetAsync(pp[0], 0, sizeof(double) * n, s));
+  TL_HIP(hipMemsetAsync(pp[1], 0, sizeof(double) * n, s));
+  const int chunk = std::max(1, o->check_every);
+  PcgState hs{};
+  int64_t kk = 0;
+  for (int launched = 0;;) {
+    for (int k = 0; k < chunk; ++k) {
+      // z = M r ; rho = r.z ; beta
+      if (o->precond == TL_PRECOND_GMG) {
+        TL_TRY(mg_vcycle(P, r, z, done));
+      } else {
+        hipLaunchKernelGGL(jacobi_prec_kernel, dim3(rb), dim3(BS), 0, s, r,
+                           o->precond == TL_PRECOND_JACOBI ? P->op.dinv64.as<double>() : nullptr,
+                           z, n, partials, counter, st, done);
+        TL_CHECK_LAUNCH();
+      }
+      // p = z + beta p ; q = K p ; pq ; alpha
+      double* pin = pp[kk & 1];
+      double* pout = pp[(kk + 1) & 1];
+      ++kk;
+      TL_TRY(fine_apply_ex(P, z, q, TL_PREC_FP64, KU_STORE_DOT, nullptr, false, nullptr, 0.0, pin,
+                           pout, &st->beta, nullptr, FIN_PQ, done, false));
+      // x += alpha p ; r -= alpha q ; rr ; done
+      hipLaunchKernelGGL(update_kernel, dim3(rb), dim3(BS), 0, s, u, r, pout, q, n, partials,
+                         counter, st, done);
+      TL_CHECK_LAUNCH();
+    }
+    launched += chunk;
+    TL_HIP(hipMemcpyAsync(&hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    TL_HIP(hipStreamSynchronize(s));
+    if (hs.done) break;
+    if (launched > o->maxiter + chunk) break;  // defensive: done must have been set
+  }
+  // true residual in fp64
+  TL_TRY(fine_apply_ex(P, u, q, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0, nullptr,
+                       nullptr, nullptr, nullptr, FIN_STORE, nullptr, true));
+  double* scal = P->partials.as<double>() + 3 * kMaxRedBlocks;  // scratch slot for the total
+  hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn,
+                     static_cast<double*>(nullptr), n, partials, counter, FIN_STORE, st, scal);
+  TL_CHECK_LAUNCH();
+  double tr2 = 0.0;
+  TL_HIP(hipMemcpyAsync(&tr2, scal, sizeof(double), hipMemcpyDeviceToHost, s));
+  TL_HIP(hipMemcpyAsync(&hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  TL_HIP(hipStreamSynchronize(s));
+  const bool conv = hs.rr < hs.tol2;
+  if (info) {
+    const double bn = std::sqrt(hs.bb);
+    info->iterations = hs.it;
+    info->status = conv ? 0 : TL_ENOCONV;
+    info->relres_recursive = bn > 0 ? std::sqrt(hs.rr) / bn : 0.0;
+    info->relres_true = bn > 0 ? std::sqrt(tr2) / bn : 0.0;
+  }
+  if (!conv) {
+    set_error("pcg iteration did not converge");
+    return TL_ENOCONV;
+  }
+  return TL_OK;
+}
+
+}  // extern "C"

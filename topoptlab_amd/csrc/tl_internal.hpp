@@ -1,0 +1,260 @@
+// Internal declarations shared by the topoptlab_amd HIP sources (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/topoptlab_amd.h"
+
+namespace tl {
+
+// ----------------------------------------------------------------------------
+// error handling
+// ----------------------------------------------------------------------------
+void set_error(const std::string& msg);
+
+struct Status {
+  int code = TL_OK;
+};
+
+#define TL_HIP(expr)                                                         \
+  do {                                                                       \
+    hipError_t _e = (expr);                                                  \
+    if (_e != hipSuccess) {                                                  \
+      ::tl::set_error(std::string(#expr) + ": " + hipGetErrorString(_e));   \
+      return TL_EHIP;                                                        \
+    }                                                                        \
+  } while (0)
+
+#define TL_CHECK_LAUNCH() TL_HIP(hipGetLastError())
+
+#define TL_ARG(cond, msg)              \
+  do {                                 \
+    if (!(cond)) {                     \
+      ::tl::set_error(msg);            \
+      return TL_EINVAL;                \
+    }                                  \
+  } while (0)
+
+#define TL_TRY(expr)               \
+  do {                             \
+    int _rc = (expr);              \
+    if (_rc != TL_OK) return _rc;  \
+  } while (0)
+
+// ----------------------------------------------------------------------------
+// device buffers owned by a handle
+// ----------------------------------------------------------------------------
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  int alloc(size_t nbytes);
+  void release();
+  template <typename T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+// ----------------------------------------------------------------------------
+// block reductions with a deterministic last-block finish
+// ----------------------------------------------------------------------------
+constexpr int kMaxRedBlocks = 4096;  // partial slots per reduction value
+
+// Sum of v over the 64 lanes of the calling wavefront (fixed butterfly order).
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+
+// Block-wide sum of NV values per thread; result valid in every thread.
+// sh must hold NV * (blockDim.x/64) doubles.
+template <int NV>
+__device__ inline void block_sum(double (&v)[NV], double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+#pragma unroll
+  for (int k = 0; k < NV; ++k) v[k] = wave_sum(v[k]);
+  __syncthreads();
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k) sh[k * nw + wid] = v[k];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) {
+    double s = 0.0;
+    for (int w = 0; w < nw; ++w) s += sh[k * nw + w];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+// Each block calls this with its block total (valid in all threads).  The
+// block that arrives last (there is exactly one) gets true and, in every
+// thread, tot[k] = sum over blocks in block-index order (deterministic).
+// partials: NV*kMaxRedBlocks doubles; counter: one zero-initialised uint,
+// reset to zero by the last block.
+template <int NV>
+__device__ inline bool last_block_total(const double (&part)[NV],
+                                        double* partials, unsigned* counter,
+                                        double (&tot)[NV], double* sh) {
+  __shared__ unsigned s_ticket;
+  const unsigned nb = gridDim.x * gridDim.y * gridDim.z;
+  const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < NV; ++k)
+      __hip_atomic_store(&partials[k * kMaxRedBlocks + bid], part[k],
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence();
+    s_ticket = atomicAdd(counter, 1u);
+  }
+  __syncthreads();
+  if (s_ticket != nb - 1) return false;
+  __threadfence();
+  // one wavefront folds the partials in a fixed order
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  if (threadIdx.x < 64) {
+    for (unsigned b = threadIdx.x; b < nb; b += 64) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k)
+        acc[k] += __hip_atomic_load(&partials[k * kMaxRedBlocks + b],
+                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+#pragma unroll
+    for (int k = 0; k < NV; ++k) acc[k] = wave_sum(acc[k]);
+    if (threadIdx.x == 0) {
+#pragma unroll
+      for (int k = 0; k < NV; ++k) sh[k] = acc[k];
+      *counter = 0u;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NV; ++k) tot[k] = sh[k];
+  return true;
+}
+
+// ----------------------------------------------------------------------------
+// structured-grid geometry
+// ----------------------------------------------------------------------------
+// Local node k of a Q4/H8 element (reference order, bottom layer first):
+// k = 4*dz + {(0,1)->0, (1,1)->1, (1,0)->2, (0,0)->3}[(dx,dy)], with (dx,dy,dz)
+// the node's offset from the element's (ix,iy,iz) corner node.
+__host__ __device__ constexpr int loc_dx(int k) { return ((k & 3) == 1 || (k & 3) == 2) ? 1 : 0; }
+__host__ __device__ constexpr int loc_dy(int k) { return ((k & 3) <= 1) ? 1 : 0; }
+__host__ __device__ constexpr int loc_dz(int k) { return k >> 2; }
+__host__ __device__ constexpr int loc_of(int dx, int dy, int dz) {
+  return 4 * dz + (dy == 1 ? dx : 3 - dx);
+}
+
+struct Grid {
+  int dim = 3;
+  int nelx = 0, nely = 0, nelz = 0;  // nelz = 0 for 2D
+  int nx = 1, ny = 1, nz = 1;        // node counts
+  int64_t nnodes = 0, nel = 0;
+  void set(int d, int ex, int ey, int ez) {
+    dim = d; nelx = ex; nely = ey; nelz = (d == 3) ? ez : 0;
+    nx = ex + 1; ny = ey + 1; nz = (d == 3) ? ez + 1 : 1;
+    nnodes = int64_t(nx) * ny * nz;
+    nel = int64_t(ex) * ey * (d == 3 ? ez : 1);
+  }
+  int64_t npad() const {  // padded element array (zero border)
+    return int64_t(nelx + 2) * (nely + 2) * (dim == 3 ? nelz + 2 : 1);
+  }
+};
+
+// ----------------------------------------------------------------------------
+// problem handle
+// ----------------------------------------------------------------------------
+struct PcgState {
+  double rho, rho_prev, pq, rr, alpha, beta, tol2, bb;
+  int it, maxiter, done, pad;
+};
+
+struct Level {
+  Grid g;
+  int64_t ndof = 0;
+  DevBuf mask;            // uint8 per node, bit d = DOF d fixed
+  DevBuf st64, st32;      // coarse stencil [ns*ndpn*ndpn][nnodes]
+  DevBuf dinv64, dinv32;  // 1/diag (0 on fixed DOFs)
+  DevBuf em;              // element matrices (levels >= 1, fp64)
+  DevBuf vb, vx, vr, vt;  // V-cycle work vectors (V-cycle precision)
+};
+
+struct MgState {
+  bool ready = false;
+  int nlevels = 0, nu1 = 1, nu2 = 1, prec = TL_PREC_FP32;
+  double omega = 0.6;
+  std::vector<Level> lv;  // lv[0] is the fine grid (no stencil)
+  DevBuf G1;              // 8 x (nd*nd) fp64: child Galerkin matrices
+  DevBuf W;               // 8 x 8 fp64: child interpolation weights
+  DevBuf cinv;            // dense inverse of the coarsest operator (fp64)
+  int64_t cn = 0, cnp = 0;  // coarsest ndof, padded size
+};
+
+struct FineOp {
+  DevBuf KE64, KE32, KE16;     // element matrix (fp64, fp32, reserved)
+  DevBuf sp64, sp32;           // padded SIMP scale
+  DevBuf dinv64, dinv32;       // 1/diag(K_ff)
+  DevBuf mask;                 // uint8 per node
+  bool have_density = false;
+};
+
+}  // namespace tl
+
+namespace tl {
+struct ProfEvent {
+  hipEvent_t start, stop;
+  int kind;
+};
+}  // namespace tl
+
+struct tl_problem {
+  tl::Grid g;
+  int ndpn = 1, nd = 8;   // DOFs per node, per element
+  int64_t ndof = 0;
+  double eps = 1e-9, penal = 3.0;
+  hipStream_t stream = nullptr;
+  std::vector<double> KEh;
+  std::vector<int64_t> fixed_h;
+  tl::FineOp op;
+  tl::MgState mg;
+  // PCG workspace (fp64, ndof each) and reduction scratch
+  tl::DevBuf r, z, p, p2, q, t;
+  tl::DevBuf partials, counter, state;
+  // launch timing of the fine operator (tl_profile)
+  bool prof_on = false;
+  std::vector<tl::ProfEvent> prof_used, prof_free;
+  ~tl_problem();
+};
+
+namespace tl {
+// operator entry points shared between translation units
+int fine_apply_ex(tl_problem* P, const void* u, void* y, int prec, int mode, const void* b,
+                  bool b64, const void* dinv, double omega, const void* zfuse_pin,
+                  void* pout, const double* beta, double* dot_out, int fin, const int* done,
+                  bool out64);
+int fine_diag(tl_problem* P);
+int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done);
+}  // namespace tl
+
+// modes of the fine matrix-free operator epilogue
+enum {
+  KU_STORE = 0,       // y = K u
+  KU_STORE_DOT = 1,   // y = K u ; dot += u.y  (pcg: p = z + beta p fused)
+  KU_RESID = 2,       // y = b - K u
+  KU_JACOBI = 3,      // y = u + omega*dinv*(b - K u)
+  KU_JACOBI_DOT = 4,  // as KU_JACOBI, y written fp64, dot += b.y
+};

@@ -1,0 +1,786 @@
+// Geometric multigrid on structured grids: Galerkin coarse operators stored
+// as nodal stencils, stride-2 (bi/tri)linear transfers, damped-Jacobi V-cycle
+// and a dense inverse on the coarsest grid.
+//
+// Reference: topoptlab/multigrid.py:89-223 (V-cycle with Galerkin P^T A P and
+// a direct coarsest solve), topoptlab/gmg.py:34-224 (shape-function
+// prolongation), linear_solvers.py:306-372 (damped Jacobi).  Deviations kept
+// on purpose: zero initial guess on coarse levels (symmetric V-cycle), coarse
+// operators built once per density instead of once per cycle, no absolute
+// early exit inside the smoother.
+//
+// Coarse operators: with P_l the nodal interpolation (x) I_ndpn and Dirichlet
+// sets that are grid aligned (every fixed DOF interpolates only from fixed
+// coarse DOFs -- checked on the host), the Galerkin operator of K_ff equals
+// M_{l} P^T A_{l-1} P M_{l} with the fixed rows/cols of level l removed.  Level 1
+// is assembled straight from the fine densities: coarse element matrix =
+// sum_children s_child * G_c with G_c = Q_c^T KE Q_c (8 constant matrices).
+// Deeper levels are Galerkin products of the previous level's stencil.
+#include "tl_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <type_traits>
+
+namespace tl {
+
+namespace {
+
+constexpr int BS = 256;
+constexpr int GJB = 32;  // Gauss-Jordan block size
+
+__host__ __device__ inline int nstencil(int dim) { return dim == 3 ? 27 : 9; }
+__host__ __device__ inline int center_k(int dim) { return dim == 3 ? 13 : 4; }
+
+struct GridD {  // device-side copy of a level grid
+  int dim, nelx, nely, nelz, nx, ny, nz;
+  int64_t nnodes;
+};
+
+GridD to_d(const Grid& g) {
+  return GridD{g.dim, g.nelx, g.nely, g.nelz, g.nx, g.ny, g.nz, g.nnodes};
+}
+
+__device__ inline void node_xyz(const GridD& g, int64_t n, int& ix, int& iy, int& iz) {
+  iy = int(n % g.ny);
+  const int64_t t = n / g.ny;
+  ix = int(t % g.nx);
+  iz = int(t / g.nx);
+}
+
+__device__ inline int64_t node_id(const GridD& g, int ix, int iy, int iz) {
+  return (int64_t(iz) * g.nx + ix) * g.ny + iy;
+}
+
+__device__ inline bool node_in(const GridD& g, int ix, int iy, int iz) {
+  return ix >= 0 && ix < g.nx && iy >= 0 && iy < g.ny && iz >= 0 && iz < g.nz;
+}
+
+// stencil index of a relative offset (rx, ry, rz)
+__device__ inline int kidx(int dim, int rx, int ry, int rz) {
+  return dim == 3 ? ((rz + 1) * 3 + (rx + 1)) * 3 + (ry + 1) : (rx + 1) * 3 + (ry + 1);
+}
+
+// ---------------------------------------------------------------------------
+// level-1 stencil straight from the fine densities
+// ---------------------------------------------------------------------------
+__global__ void st_from_fine_kernel(GridD g1, const double* __restrict__ spf, int fnelx,
+                                    int fnely, const double* __restrict__ G1, int ndpn,
+                                    const uint8_t* __restrict__ mask1, double* __restrict__ st) {
+  const int ns = nstencil(g1.dim);
+  const int nq = ns * ndpn * ndpn;
+  const int nd = (g1.dim == 3 ? 8 : 4) * ndpn;
+  const int nch = g1.dim == 3 ? 8 : 4;
+  const int64_t tot = g1.nnodes * nq;
+  for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
+       t += int64_t(gridDim.x) * BS) {
+    const int64_t n = t % g1.nnodes;
+    const int q = int(t / g1.nnodes);
+    const int c = q % ndpn, d = (q / ndpn) % ndpn, k = q / (ndpn * ndpn);
+    int rx, ry, rz;
+    if (g1.dim == 3) { rz = k / 9 - 1; rx = (k / 3) % 3 - 1; ry = k % 3 - 1; }
+    else { rz = 0; rx = k / 3 - 1; ry = k % 3 - 1; }
+    int ix, iy, iz;
+    node_xyz(g1, n, ix, iy, iz);
+    double acc = 0.0;
+    const bool mexists = node_in(g1, ix + rx, iy + ry, iz + rz);
+    if (mexists) {
+      for (int oz = (g1.dim == 3 ? -1 : 0); oz <= 0; ++oz)
+        for (int ox = -1; ox <= 0; ++ox)
+          for (int oy = -1; oy <= 0; ++oy) {
+            const int ex = ix + ox, ey = iy + oy, ez = iz + oz;
+            if (ex < 0 || ex >= g1.nelx || ey < 0 || ey >= g1.nely) continue;
+            if (g1.dim == 3 && (ez < 0 || ez >= g1.nelz)) continue;
+            const int bx = rx - ox, by = ry - oy, bz = rz - oz;
+            if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+            const int la = loc_of(-ox, -oy, -oz), lb = loc_of(bx, by, bz);
+            const int ent = (la * ndpn + d) * nd + lb * ndpn + c;
+            for (int ch = 0; ch < nch; ++ch) {
+              const int cy = ch & 1, cx = (ch >> 1) & 1, cz = ch >> 2;
+              const int fx = 2 * ex + cx, fy = 2 * ey + cy, fz = 2 * ez + cz;
+              const int64_t pidx =
+                  (g1.dim == 3 ? (int64_t(fz + 1) * (fnelx + 2) + fx + 1) * (fnely + 2)
+                               : int64_t(fx + 1) * (fnely + 2)) + fy + 1;
+              acc += spf[pidx] * G1[ch * nd * nd + ent];
+            }
+          }
+    }
+    const bool fn = (mask1[n] >> d) & 1;
+    const bool fm = mexists ? ((mask1[node_id(g1, ix + rx, iy + ry, iz + rz)] >> c) & 1) : true;
+    if (fn || fm) acc = (fn && rx == 0 && ry == 0 && rz == 0 && c == d) ? 1.0 : 0.0;
+    st[int64_t(q) * g1.nnodes + n] = acc;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// stencil Galerkin: level l (fine) -> level l+1 (coarse)
+// ---------------------------------------------------------------------------
+__global__ void st_coarsen_kernel(GridD gf, GridD gc, int ndpn, const double* __restrict__ stf,
+                                  const uint8_t* __restrict__ maskc, double* __restrict__ stc) {
+  const int ns = nstencil(gc.dim);
+  const int nq = ns * ndpn * ndpn;
+  const int64_t tot = gc.nnodes * nq;
+  const bool d3 = gc.dim == 3;
+  for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
+       t += int64_t(gridDim.x) * BS) {
+    const int64_t n = t % gc.nnodes;
+    const int q = int(t / gc.nnodes);
+    const int c = q % ndpn, d = (q / ndpn) % ndpn, k = q / (ndpn * ndpn);
+    int rx, ry, rz;
+    if (d3) { rz = k / 9 - 1; rx = (k / 3) % 3 - 1; ry = k % 3 - 1; }
+    else { rz = 0; rx = k / 3 - 1; ry = k % 3 - 1; }
+    int ix, iy, iz;
+    node_xyz(gc, n, ix, iy, iz);
+    double acc = 0.0;
+    const bool mexists = node_in(gc, ix + rx, iy + ry, iz + rz);
+    if (mexists) {
+      for (int dz = (d3 ? -1 : 0); dz <= (d3 ? 1 : 0); ++dz)
+        for (int dx = -1; dx <= 1; ++dx)
+          for (int dy = -1; dy <= 1; ++dy) {
+            const int fx = 2 * ix + dx, fy = 2 * iy + dy, fz = d3 ? 2 * iz + dz : 0;
+            if (!node_in(gf, fx, fy, fz)) continue;
+            const double wi = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+            const int64_t fi = node_id(gf, fx, fy, fz);
+            for (int sz = (d3 ? -1 : 0); sz <= (d3 ? 1 : 0); ++sz) {
+              const int ez = dz + sz - 2 * rz;
+              if (ez < -1 || ez > 1) continue;
+              for (int sx = -1; sx <= 1; ++sx) {
+                const int ex = dx + sx - 2 * rx;
+                if (ex < -1 || ex > 1) continue;
+                for (int sy = -1; sy <= 1; ++sy) {
+                  const int ey = dy + sy - 2 * ry;
+                  if (ey < -1 || ey > 1) continue;
+                  const double wj = (ex ? 0.5 : 1.0) * (ey ? 0.5 : 1.0) * (ez ? 0.5 : 1.0);
+                  const int ks = kidx(gf.dim, sx, sy, sz);
+                  acc += wi * wj * stf[int64_t((ks * ndpn + d) * ndpn + c) * gf.nnodes + fi];
+                }
+              }
+            }
+          }
+    }
+    const bool fn = (maskc[n] >> d) & 1;
+    const bool fm = mexists ? ((maskc[node_id(gc, ix + rx, iy + ry, iz + rz)] >> c) & 1) : true;
+    if (fn || fm) acc = (fn && rx == 0 && ry == 0 && rz == 0 && c == d) ? 1.0 : 0.0;
+    stc[int64_t(q) * gc.nnodes + n] = acc;
+  }
+}
+
+// fp32 copy of the stencil and inverse diagonal
+__global__ void st_finish_kernel(GridD g, int ndpn, const double* __restrict__ st64,
+                                 float* __restrict__ st32, const uint8_t* __restrict__ mask,
+                                 double* __restrict__ dinv64, float* __restrict__ dinv32) {
+  const int nq = nstencil(g.dim) * ndpn * ndpn;
+  const int64_t tot = g.nnodes * nq;
+  for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
+       t += int64_t(gridDim.x) * BS)
+    st32[t] = float(st64[t]);
+  const int kc = center_k(g.dim);
+  for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < g.nnodes * ndpn;
+       t += int64_t(gridDim.x) * BS) {
+    const int64_t n = t / ndpn;
+    const int d = int(t % ndpn);
+    const double dg = st64[int64_t((kc * ndpn + d) * ndpn + d) * g.nnodes + n];
+    const bool fixed = (mask[n] >> d) & 1;
+    const double inv = (fixed || dg == 0.0) ? 0.0 : 1.0 / dg;
+    dinv64[t] = inv;
+    dinv32[t] = float(inv);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// V-cycle kernels (T = float or double)
+// ---------------------------------------------------------------------------
+// y = b - A x  (MODE 0)   or   y = x + omega*dinv*(b - A x)  (MODE 1)
+template <int NDPN, typename T, int MODE>
+__global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restrict__ st,
+                                                      const T* __restrict__ x,
+                                                      const T* __restrict__ b,
+                                                      const T* __restrict__ dinv, T omega,
+                                                      const uint8_t* __restrict__ mask,
+                                                      T* __restrict__ y, const int* done) {
+  if (*done) return;
+  const int64_t n = blockIdx.x * int64_t(BS) + threadIdx.x;
+  if (n >= g.nnodes) return;
+  int ix, iy, iz;
+  node_xyz(g, n, ix, iy, iz);
+  T acc[NDPN];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
+  const bool d3 = g.dim == 3;
+  for (int rz = (d3 ? -1 : 0); rz <= (d3 ? 1 : 0); ++rz)
+#pragma unroll
+    for (int rx = -1; rx <= 1; ++rx)
+#pragma unroll
+      for (int ry = -1; ry <= 1; ++ry) {
+        const int k = kidx(g.dim, rx, ry, rz);
+        // neighbours outside the grid carry zero stencil weight: clamp the read
+        const int mx = min(max(ix + rx, 0), g.nx - 1), my = min(max(iy + ry, 0), g.ny - 1),
+                  mz = min(max(iz + rz, 0), g.nz - 1);
+        const int64_t m = node_id(g, mx, my, mz);
+        T xm[NDPN];
+#pragma unroll
+        for (int c = 0; c < NDPN; ++c) xm[c] = x[m * NDPN + c];
+#pragma unroll
+        for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+          for (int c = 0; c < NDPN; ++c)
+            acc[d] += st[int64_t((k * NDPN + d) * NDPN + c) * g.nnodes + n] * xm[c];
+      }
+  const uint8_t mk = mask[n];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) {
+    const int64_t i = n * NDPN + d;
+    T v;
+    if (MODE == 0) v = b[i] - acc[d];
+    else v = x[i] + omega * dinv[i] * (b[i] - acc[d]);
+    y[i] = ((mk >> d) & 1) ? T(0) : v;
+  }
+}
+
+// x = omega*dinv*b  (first smoothing sweep from a zero guess); b may be fp64
+template <typename T, typename TB>
+__global__ void jacobi_zero_kernel(const TB* __restrict__ b, const T* __restrict__ dinv, T omega,
+                                   T* __restrict__ x, int64_t n, const int* done) {
+  if (*done) return;
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    x[i] = omega * dinv[i] * T(b[i]);
+}
+
+// coarse b = P^T r (masked)
+template <int NDPN, typename T>
+__global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
+                                const uint8_t* __restrict__ maskc, T* __restrict__ bc,
+                                const int* done) {
+  if (*done) return;
+  const int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (n >= gc.nnodes) return;
+  int ix, iy, iz;
+  node_xyz(gc, n, ix, iy, iz);
+  T acc[NDPN];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
+  const bool d3 = gc.dim == 3;
+  for (int dz = (d3 ? -1 : 0); dz <= (d3 ? 1 : 0); ++dz)
+    for (int dx = -1; dx <= 1; ++dx)
+      for (int dy = -1; dy <= 1; ++dy) {
+        const int fx = 2 * ix + dx, fy = 2 * iy + dy, fz = d3 ? 2 * iz + dz : 0;
+        if (!node_in(gf, fx, fy, fz)) continue;
+        const T w = T((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
+        const int64_t fi = node_id(gf, fx, fy, fz);
+#pragma unroll
+        for (int d = 0; d < NDPN; ++d) acc[d] += w * rf[fi * NDPN + d];
+      }
+  const uint8_t mk = maskc[n];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) bc[n * NDPN + d] = ((mk >> d) & 1) ? T(0) : acc[d];
+}
+
+// fine x += P xc (masked)
+template <int NDPN, typename T>
+__global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
+                                   const uint8_t* __restrict__ maskf, T* __restrict__ xf,
+                                   const int* done) {
+  if (*done) return;
+  const int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (n >= gf.nnodes) return;
+  int ix, iy, iz;
+  node_xyz(gf, n, ix, iy, iz);
+  T acc[NDPN];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
+  const int x0 = ix >> 1, x1 = (ix + 1) >> 1, y0 = iy >> 1, y1 = (iy + 1) >> 1;
+  const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
+  const T wx = (ix & 1) ? T(0.5) : T(1), wy = (iy & 1) ? T(0.5) : T(1), wz = (iz & 1) ? T(0.5) : T(1);
+  for (int cz = z0; cz <= z1; ++cz)
+    for (int cx = x0; cx <= x1; ++cx)
+      for (int cy = y0; cy <= y1; ++cy) {
+        const T w = wx * wy * wz;
+        const int64_t ci = node_id(gc, cx, cy, cz);
+#pragma unroll
+        for (int d = 0; d < NDPN; ++d) acc[d] += w * xc[ci * NDPN + d];
+      }
+  const uint8_t mk = maskf[n];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d)
+    if (!((mk >> d) & 1)) xf[n * NDPN + d] += acc[d];
+}
+
+// x = Cinv b on the coarsest grid (Cinv fp64, padded leading dimension)
+template <typename T>
+__global__ __launch_bounds__(BS) void dense_gemv_kernel(const double* __restrict__ A, int64_t np,
+                                                        int64_t n, const T* __restrict__ b,
+                                                        T* __restrict__ x, const int* done) {
+  if (*done) return;
+  const int64_t row = blockIdx.x * int64_t(BS / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (row >= n) return;
+  double s = 0.0;
+  for (int64_t j = lane; j < n; j += 64) s += A[row * np + j] * double(b[j]);
+  s = wave_sum(s);
+  if (lane == 0) x[row] = T(s);
+}
+
+// ---------------------------------------------------------------------------
+// dense coarsest operator and its in-place blocked Gauss-Jordan inverse
+// ---------------------------------------------------------------------------
+__global__ void dense_from_stencil_kernel(GridD g, int ndpn, const double* __restrict__ st,
+                                          double* __restrict__ A, int64_t np) {
+  const int nq = nstencil(g.dim) * ndpn * ndpn;
+  const int64_t tot = g.nnodes * nq;
+  for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
+       t += int64_t(gridDim.x) * BS) {
+    const int64_t n = t % g.nnodes;
+    const int q = int(t / g.nnodes);
+    const int c = q % ndpn, d = (q / ndpn) % ndpn, k = q / (ndpn * ndpn);
+    int rx, ry, rz;
+    if (g.dim == 3) { rz = k / 9 - 1; rx = (k / 3) % 3 - 1; ry = k % 3 - 1; }
+    else { rz = 0; rx = k / 3 - 1; ry = k % 3 - 1; }
+    int ix, iy, iz;
+    node_xyz(g, n, ix, iy, iz);
+    if (!node_in(g, ix + rx, iy + ry, iz + rz)) continue;
+    const int64_t m = node_id(g, ix + rx, iy + ry, iz + rz);
+    A[(n * ndpn + d) * np + m * ndpn + c] = st[t];
+  }
+}
+
+__global__ void pad_identity_kernel(double* A, int64_t n, int64_t np) {
+  const int64_t i = n + blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (i < np) A[i * np + i] = 1.0;
+}
+
+// step k, part 1: invert the diagonal GJB x GJB block in place
+__global__ __launch_bounds__(GJB* GJB) void gj_pivot_kernel(double* A, int64_t np, int k) {
+  __shared__ double a[GJB][GJB + 1];
+  const int i = threadIdx.x / GJB, j = threadIdx.x % GJB;
+  const int64_t kb = int64_t(k) * GJB;
+  a[i][j] = A[(kb + i) * np + kb + j];
+  __syncthreads();
+  for (int t = 0; t < GJB; ++t) {
+    const double att = a[t][t], ait = a[i][t], atj = a[t][j], aij = a[i][j];
+    double nv;
+    if (i == t && j == t) nv = 1.0 / att;
+    else if (i == t) nv = atj / att;
+    else if (j == t) nv = -ait / att;
+    else nv = aij - ait * atj / att;
+    __syncthreads();
+    a[i][j] = nv;
+    __syncthreads();
+  }
+  A[(kb + i) * np + kb + j] = a[i][j];
+}
+
+// part 2: row panel A[kb:kb+B, j] <- Pinv * A[kb:kb+B, j] for j outside the block
+__global__ __launch_bounds__(BS) void gj_rowpanel_kernel(double* A, int64_t np, int k) {
+  __shared__ double pv[GJB][GJB];
+  const int64_t kb = int64_t(k) * GJB;
+  for (int t = threadIdx.x; t < GJB * GJB; t += BS) pv[t / GJB][t % GJB] = A[(kb + t / GJB) * np + kb + t % GJB];
+  __syncthreads();
+  const int64_t j = blockIdx.x * int64_t(BS) + threadIdx.x;
+  if (j >= np || (j >= kb && j < kb + GJB)) return;
+  double col[GJB];
+#pragma unroll
+  for (int t = 0; t < GJB; ++t) col[t] = A[(kb + t) * np + j];
+#pragma unroll 4
+  for (int r = 0; r < GJB; ++r) {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < GJB; ++t) s += pv[r][t] * col[t];
+    A[(kb + r) * np + j] = s;
+  }
+}
+
+// part 3: A[i,j] -= A[i, kb:kb+B] * A[kb:kb+B, j] for i, j outside the block
+// (column panel still old, row panel already updated); 64x64 tiles
+__global__ __launch_bounds__(BS) void gj_update_kernel(double* A, int64_t np, int k) {
+  __shared__ double cp[64][GJB + 1];
+  __shared__ double rp[GJB][64 + 1];
+  const int64_t kb = int64_t(k) * GJB;
+  const int64_t i0 = int64_t(blockIdx.y) * 64, j0 = int64_t(blockIdx.x) * 64;
+  for (int t = threadIdx.x; t < 64 * GJB; t += BS) {
+    const int r = t / GJB, c = t % GJB;
+    cp[r][c] = (i0 + r < np) ? A[(i0 + r) * np + kb + c] : 0.0;
+    const int r2 = t / 64, c2 = t % 64;
+    rp[r2][c2] = (j0 + c2 < np) ? A[(kb + r2) * np + j0 + c2] : 0.0;
+  }
+  __syncthreads();
+  const int tj = threadIdx.x % 16, ti = threadIdx.x / 16;  // 16 x 16 threads, 4x4 each
+  double acc[4][4] = {};
+#pragma unroll 8
+  for (int t = 0; t < GJB; ++t) {
+    double av[4], bv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) { av[u] = cp[ti * 4 + u][t]; bv[u] = rp[t][tj * 4 + u]; }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t i = i0 + ti * 4 + u;
+    if (i >= np || (i >= kb && i < kb + GJB)) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int64_t j = j0 + tj * 4 + v;
+      if (j >= np || (j >= kb && j < kb + GJB)) continue;
+      A[i * np + j] -= acc[u][v];
+    }
+  }
+}
+
+// part 4: column panel A[i, kb:kb+B] <- -A[i, kb:kb+B] * Pinv for i outside the block
+__global__ __launch_bounds__(BS) void gj_colpanel_kernel(double* A, int64_t np, int k) {
+  __shared__ double pv[GJB][GJB];
+  const int64_t kb = int64_t(k) * GJB;
+  for (int t = threadIdx.x; t < GJB * GJB; t += BS) pv[t / GJB][t % GJB] = A[(kb + t / GJB) * np + kb + t % GJB];
+  __syncthreads();
+  const int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x;
+  if (i >= np || (i >= kb && i < kb + GJB)) return;
+  double row[GJB];
+#pragma unroll
+  for (int t = 0; t < GJB; ++t) row[t] = A[i * np + kb + t];
+#pragma unroll 4
+  for (int c = 0; c < GJB; ++c) {
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < GJB; ++t) s += row[t] * pv[t][c];
+    A[i * np + kb + c] = -s;
+  }
+}
+
+unsigned gsz(int64_t work) {
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>((work + BS - 1) / BS, 1 << 20)));
+}
+
+// ---------------------------------------------------------------------------
+// host helpers
+// ---------------------------------------------------------------------------
+// Child interpolation W_c[i][j]: weight of coarse local node j at fine local
+// node i of child c (child offset (cx,cy,cz), c = (cz*2+cx)*2+cy).
+void child_weights(int dim, std::vector<double>& W) {
+  const int nn = dim == 3 ? 8 : 4, nch = nn;
+  W.assign(size_t(nch) * nn * nn, 0.0);
+  for (int ch = 0; ch < nch; ++ch) {
+    const int cy = ch & 1, cx = (ch >> 1) & 1, cz = ch >> 2;
+    for (int i = 0; i < nn; ++i) {
+      const double px = 0.5 * (cx + loc_dx(i)), py = 0.5 * (cy + loc_dy(i)),
+                   pz = 0.5 * (cz + loc_dz(i));
+      for (int j = 0; j < nn; ++j) {
+        double w = (loc_dx(j) ? px : 1 - px) * (loc_dy(j) ? py : 1 - py);
+        if (dim == 3) w *= (loc_dz(j) ? pz : 1 - pz);
+        W[(size_t(ch) * nn + i) * nn + j] = w;
+      }
+    }
+  }
+}
+
+// per-node fixed bit masks of every level; fails if the Dirichlet set is not
+// grid aligned (a fixed DOF interpolated from a free coarse DOF)
+int level_masks(const tl_problem* P, int nlevels, std::vector<std::vector<uint8_t>>& masks) {
+  masks.assign(nlevels, {});
+  masks[0].assign(P->g.nnodes, 0);
+  for (int64_t f : P->fixed_h) masks[0][f / P->ndpn] |= uint8_t(1u << (f % P->ndpn));
+  Grid gf = P->g;
+  for (int l = 1; l < nlevels; ++l) {
+    Grid gc;
+    gc.set(gf.dim, gf.nelx / 2, gf.nely / 2, gf.nelz / 2);
+    auto& mc = masks[l];
+    const auto& mf = masks[l - 1];
+    mc.assign(gc.nnodes, 0);
+    for (int iz = 0; iz < gc.nz; ++iz)
+      for (int ix = 0; ix < gc.nx; ++ix)
+        for (int iy = 0; iy < gc.ny; ++iy)
+          mc[(int64_t(iz) * gc.nx + ix) * gc.ny + iy] =
+              mf[(int64_t(2 * iz) * gf.nx + 2 * ix) * gf.ny + 2 * iy];
+    for (int iz = 0; iz < gf.nz; ++iz)
+      for (int ix = 0; ix < gf.nx; ++ix)
+        for (int iy = 0; iy < gf.ny; ++iy) {
+          const uint8_t m = mf[(int64_t(iz) * gf.nx + ix) * gf.ny + iy];
+          if (!m) continue;
+          for (int cz = iz >> 1; cz <= (iz + 1) >> 1; ++cz)
+            for (int cx = ix >> 1; cx <= (ix + 1) >> 1; ++cx)
+              for (int cy = iy >> 1; cy <= (iy + 1) >> 1; ++cy) {
+                const uint8_t pc = mc[(int64_t(cz) * gc.nx + cx) * gc.ny + cy];
+                if ((m & pc) != m) {
+                  set_error("multigrid needs grid-aligned Dirichlet DOFs: a fixed DOF "
+                            "interpolates from a free coarse DOF");
+                  return TL_EINVAL;
+                }
+              }
+        }
+    gf = gc;
+  }
+  return TL_OK;
+}
+
+template <int NDPN, typename T>
+int vcycle_level(tl_problem* P, int l, const int* done);
+
+template <typename T>
+T* vp(const DevBuf& b) { return static_cast<T*>(b.ptr); }
+
+// Host-side guard: every buffer a V-cycle kernel dereferences must exist.
+#define TL_NONNULL(p) TL_ARG((p) != nullptr, "internal: unallocated multigrid buffer " #p)
+
+template <int NDPN, typename T>
+int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const int* done) {
+  Level& L = P->mg.lv[l];
+  TL_ARG(l >= 1 && l < P->mg.nlevels, "internal: stencil level out of range");
+  TL_NONNULL(L.st64.ptr); TL_NONNULL(L.st32.ptr); TL_NONNULL(L.dinv64.ptr);
+  TL_NONNULL(L.dinv32.ptr); TL_NONNULL(L.mask.ptr); TL_NONNULL(x); TL_NONNULL(b); TL_NONNULL(y);
+  const T* st = std::is_same<T, double>::value ? static_cast<const T*>(L.st64.ptr)
+                                                : static_cast<const T*>(L.st32.ptr);
+  const T* dinv = std::is_same<T, double>::value ? static_cast<const T*>(L.dinv64.ptr)
+                                                  : static_cast<const T*>(L.dinv32.ptr);
+  const unsigned nb = unsigned((L.g.nnodes + BS - 1) / BS);
+  if (mode == 0)
+    hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 0>), dim3(nb), dim3(BS), 0, P->stream, to_d(L.g),
+                       st, x, b, dinv, T(P->mg.omega), L.mask.as<uint8_t>(), y, done);
+  else
+    hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1>), dim3(nb), dim3(BS), 0, P->stream, to_d(L.g),
+                       st, x, b, dinv, T(P->mg.omega), L.mask.as<uint8_t>(), y, done);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+// Solve A_l x_l = b_l approximately (x_l := V-cycle(b_l), zero initial guess).
+template <int NDPN, typename T>
+int vcycle_level(tl_problem* P, int l, const int* done) {
+  MgState& M = P->mg;
+  Level& L = M.lv[l];
+  const int64_t n = L.ndof;
+  const unsigned nb = unsigned((n + BS - 1) / BS);
+  TL_NONNULL(L.vb.ptr); TL_NONNULL(L.vx.ptr); TL_NONNULL(L.vt.ptr); TL_NONNULL(L.vr.ptr);
+  TL_NONNULL(L.mask.ptr);
+  if (l == M.nlevels - 1) {
+    TL_NONNULL(M.cinv.ptr);
+    TL_ARG(M.cn == n, "internal: coarsest size mismatch");
+    hipLaunchKernelGGL(dense_gemv_kernel<T>, dim3(unsigned((n + 3) / 4)), dim3(BS), 0, P->stream,
+                       M.cinv.as<double>(), M.cnp, n, vp<T>(L.vb), vp<T>(L.vx), done);
+    TL_CHECK_LAUNCH();
+    return TL_OK;
+  }
+  T* x = vp<T>(L.vx);
+  T* t = vp<T>(L.vt);
+  const T* b = vp<T>(L.vb);
+  const T* dinv = std::is_same<T, double>::value ? static_cast<const T*>(L.dinv64.ptr)
+                                                  : static_cast<const T*>(L.dinv32.ptr);
+  hipLaunchKernelGGL((jacobi_zero_kernel<T, T>), dim3(gsz(n)), dim3(BS), 0, P->stream, b, dinv,
+                     T(M.omega), x, n, done);
+  TL_CHECK_LAUNCH();
+  for (int s = 1; s < M.nu1; ++s) {
+    TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
+    std::swap(L.vx.ptr, L.vt.ptr);
+    x = vp<T>(L.vx); t = vp<T>(L.vt);
+  }
+  TL_TRY((st_apply<NDPN, T>(P, l, 0, x, b, vp<T>(L.vr), done)));
+  Level& C = M.lv[l + 1];
+  hipLaunchKernelGGL((restrict_kernel<NDPN, T>), dim3(unsigned((C.g.nnodes + BS - 1) / BS)),
+                     dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(L.vr),
+                     C.mask.as<uint8_t>(), vp<T>(C.vb), done);
+  TL_CHECK_LAUNCH();
+  TL_TRY((vcycle_level<NDPN, T>(P, l + 1, done)));
+  hipLaunchKernelGGL((prolong_add_kernel<NDPN, T>), dim3(unsigned((L.g.nnodes + BS - 1) / BS)),
+                     dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(C.vx),
+                     L.mask.as<uint8_t>(), x, done);
+  TL_CHECK_LAUNCH();
+  for (int s = 0; s < M.nu2; ++s) {
+    TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
+    std::swap(L.vx.ptr, L.vt.ptr);
+    x = vp<T>(L.vx); t = vp<T>(L.vt);
+  }
+  (void)nb;
+  return TL_OK;
+}
+
+// Fine level: r (fp64 PCG residual) -> z (fp64), rho = r.z into the PCG state.
+template <int NDPN, typename T>
+int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done) {
+  MgState& M = P->mg;
+  TL_ARG(M.nlevels >= 2 && int(M.lv.size()) == M.nlevels, "internal: hierarchy not set up");
+  Level& L = M.lv[0];
+  Level& C = M.lv[1];
+  const int64_t n = P->ndof;
+  TL_NONNULL(L.vx.ptr); TL_NONNULL(L.vt.ptr); TL_NONNULL(L.vr.ptr); TL_NONNULL(C.vb.ptr);
+  TL_NONNULL(C.vx.ptr); TL_NONNULL(C.mask.ptr); TL_NONNULL(P->op.mask.ptr);
+  TL_NONNULL(P->op.dinv32.ptr); TL_NONNULL(P->op.dinv64.ptr); TL_NONNULL(r); TL_NONNULL(z);
+  const int prec = std::is_same<T, double>::value ? TL_PREC_FP64 : TL_PREC_FP32;
+  const void* dinv = std::is_same<T, double>::value ? P->op.dinv64.ptr : P->op.dinv32.ptr;
+  T* x = vp<T>(L.vx);
+  T* t = vp<T>(L.vt);
+  hipLaunchKernelGGL((jacobi_zero_kernel<T, double>), dim3(gsz(n)), dim3(BS), 0, P->stream, r,
+                     static_cast<const T*>(dinv), T(M.omega), x, n, done);
+  TL_CHECK_LAUNCH();
+  for (int s = 1; s < M.nu1; ++s) {
+    TL_TRY(fine_apply_ex(P, x, t, prec, KU_JACOBI, r, true, dinv, M.omega, nullptr, nullptr,
+                         nullptr, nullptr, 0, done, false));
+    std::swap(L.vx.ptr, L.vt.ptr);
+    x = vp<T>(L.vx); t = vp<T>(L.vt);
+  }
+  TL_TRY(fine_apply_ex(P, x, vp<T>(L.vr), prec, KU_RESID, r, true, nullptr, 0.0, nullptr, nullptr,
+                       nullptr, nullptr, 0, done, false));
+  hipLaunchKernelGGL((restrict_kernel<NDPN, T>), dim3(unsigned((C.g.nnodes + BS - 1) / BS)),
+                     dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(L.vr),
+                     C.mask.as<uint8_t>(), vp<T>(C.vb), done);
+  TL_CHECK_LAUNCH();
+  TL_TRY((vcycle_level<NDPN, T>(P, 1, done)));
+  // the fine grid's Dirichlet mask is the operator's (level 0 has no copy)
+  hipLaunchKernelGGL((prolong_add_kernel<NDPN, T>), dim3(unsigned((L.g.nnodes + BS - 1) / BS)),
+                     dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(C.vx),
+                     P->op.mask.as<uint8_t>(), x, done);
+  TL_CHECK_LAUNCH();
+  for (int s = 0; s + 1 < M.nu2; ++s) {
+    TL_TRY(fine_apply_ex(P, x, t, prec, KU_JACOBI, r, true, dinv, M.omega, nullptr, nullptr,
+                         nullptr, nullptr, 0, done, false));
+    std::swap(L.vx.ptr, L.vt.ptr);
+    x = vp<T>(L.vx); t = vp<T>(L.vt);
+  }
+  // last post-smoothing sweep writes z (fp64) and rho = r.z (FIN_RHO = 1)
+  return fine_apply_ex(P, x, z, prec, KU_JACOBI_DOT, r, true, dinv, M.omega, nullptr, nullptr,
+                       nullptr, nullptr, 1, done, true);
+}
+
+}  // namespace
+
+int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done) {
+  const bool f64 = P->mg.prec == TL_PREC_FP64;
+  switch (P->ndpn) {
+    case 1: return f64 ? vcycle_fine<1, double>(P, r, z, done) : vcycle_fine<1, float>(P, r, z, done);
+    case 2: return f64 ? vcycle_fine<2, double>(P, r, z, done) : vcycle_fine<2, float>(P, r, z, done);
+    case 3: return f64 ? vcycle_fine<3, double>(P, r, z, done) : vcycle_fine<3, float>(P, r, z, done);
+  }
+  set_error("unsupported DOFs per node");
+  return TL_EINVAL;
+}
+
+}  // namespace tl
+
+using namespace tl;
+
+extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
+  TL_ARG(P && o, "null argument");
+  TL_ARG(P->op.have_density, "tl_set_density must be called first");
+  TL_ARG(o->nlevels >= 2 && o->nlevels <= 12, "nlevels must be in [2, 12]");
+  TL_ARG(o->nu1 >= 1 && o->nu2 >= 1, "nu1 and nu2 must be >= 1");
+  TL_ARG(o->omega > 0.0, "omega must be positive");
+  TL_ARG(o->precision == TL_PREC_FP64 || o->precision == TL_PREC_FP32,
+         "V-cycle precision must be FP64 or FP32");
+  const int L = o->nlevels;
+  const int f = 1 << (L - 1);
+  const Grid& g = P->g;
+  TL_ARG(g.nelx % f == 0 && g.nely % f == 0 && (g.dim == 2 || g.nelz % f == 0),
+         "element counts must be divisible by 2^(nlevels-1)");
+  MgState& M = P->mg;
+  const int ndpn = P->ndpn, nd = P->nd;
+  const bool rebuild = M.nlevels != L || M.lv.empty();
+  M.nu1 = o->nu1; M.nu2 = o->nu2; M.omega = o->omega; M.prec = o->precision;
+  hipStream_t s = P->stream;
+  if (rebuild) {
+    std::vector<std::vector<uint8_t>> masks;
+    TL_TRY(level_masks(P, L, masks));
+    M.lv.clear();
+    M.lv.resize(L);
+    M.nlevels = L;
+    Grid gl = g;
+    for (int l = 0; l < L; ++l) {
+      Level& lv = M.lv[l];
+      if (l > 0) gl.set(gl.dim, gl.nelx / 2, gl.nely / 2, gl.nelz / 2);
+      lv.g = gl;
+      lv.ndof = gl.nnodes * ndpn;
+      const size_t vb = sizeof(double) * size_t(lv.ndof);
+      TL_TRY(lv.vx.alloc(vb)); TL_TRY(lv.vt.alloc(vb)); TL_TRY(lv.vr.alloc(vb)); TL_TRY(lv.vb.alloc(vb));
+      if (l == 0) continue;
+      TL_TRY(lv.mask.alloc(gl.nnodes));
+      TL_HIP(hipMemcpy(lv.mask.ptr, masks[l].data(), gl.nnodes, hipMemcpyHostToDevice));
+      const size_t nst = size_t(nstencil(gl.dim)) * ndpn * ndpn * gl.nnodes;
+      TL_TRY(lv.st64.alloc(sizeof(double) * nst));
+      TL_TRY(lv.st32.alloc(sizeof(float) * nst));
+      TL_TRY(lv.dinv64.alloc(sizeof(double) * lv.ndof));
+      TL_TRY(lv.dinv32.alloc(sizeof(float) * lv.ndof));
+    }
+    // G1[c] = (W_c x I)^T KE (W_c x I)
+    std::vector<double> W;
+    child_weights(g.dim, W);
+    const int nn = g.dim == 3 ? 8 : 4;
+    std::vector<double> G(size_t(nn) * nd * nd, 0.0);
+    std::vector<double> Q(size_t(nd) * nd), T(size_t(nd) * nd);
+    for (int ch = 0; ch < nn; ++ch) {
+      std::fill(Q.begin(), Q.end(), 0.0);
+      for (int i = 0; i < nn; ++i)
+        for (int j = 0; j < nn; ++j)
+          for (int d = 0; d < ndpn; ++d)
+            Q[size_t(i * ndpn + d) * nd + j * ndpn + d] = W[(size_t(ch) * nn + i) * nn + j];
+      for (int a = 0; a < nd; ++a)  // T = KE Q
+        for (int b = 0; b < nd; ++b) {
+          double sum = 0.0;
+          for (int k = 0; k < nd; ++k) sum += P->KEh[size_t(a) * nd + k] * Q[size_t(k) * nd + b];
+          T[size_t(a) * nd + b] = sum;
+        }
+      for (int a = 0; a < nd; ++a)  // G = Q^T T
+        for (int b = 0; b < nd; ++b) {
+          double sum = 0.0;
+          for (int k = 0; k < nd; ++k) sum += Q[size_t(k) * nd + a] * T[size_t(k) * nd + b];
+          G[(size_t(ch) * nd + a) * nd + b] = sum;
+        }
+    }
+    TL_TRY(M.G1.alloc(sizeof(double) * G.size()));
+    TL_HIP(hipMemcpy(M.G1.ptr, G.data(), sizeof(double) * G.size(), hipMemcpyHostToDevice));
+    const Level& Lc = M.lv[L - 1];
+    M.cn = Lc.ndof;
+    M.cnp = ((M.cn + GJB - 1) / GJB) * GJB;
+    TL_TRY(M.cinv.alloc(sizeof(double) * size_t(M.cnp) * M.cnp));
+  }
+  // level 1 from the fine densities
+  {
+    Level& l1 = M.lv[1];
+    const int64_t tot = l1.g.nnodes * nstencil(g.dim) * ndpn * ndpn;
+    hipLaunchKernelGGL(st_from_fine_kernel, dim3(gsz(tot)), dim3(BS), 0, s, to_d(l1.g),
+                       P->op.sp64.as<double>(), g.nelx, g.nely, M.G1.as<double>(), ndpn,
+                       l1.mask.as<uint8_t>(), l1.st64.as<double>());
+    TL_CHECK_LAUNCH();
+  }
+  for (int l = 2; l < L; ++l) {
+    Level& lf = M.lv[l - 1];
+    Level& lc = M.lv[l];
+    const int64_t tot = lc.g.nnodes * nstencil(g.dim) * ndpn * ndpn;
+    hipLaunchKernelGGL(st_coarsen_kernel, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lf.g), to_d(lc.g),
+                       ndpn, lf.st64.as<double>(), lc.mask.as<uint8_t>(), lc.st64.as<double>());
+    TL_CHECK_LAUNCH();
+  }
+  for (int l = 1; l < L; ++l) {
+    Level& lv = M.lv[l];
+    const int64_t tot = lv.g.nnodes * nstencil(g.dim) * ndpn * ndpn;
+    hipLaunchKernelGGL(st_finish_kernel, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lv.g), ndpn,
+                       lv.st64.as<double>(), lv.st32.as<float>(), lv.mask.as<uint8_t>(),
+                       lv.dinv64.as<double>(), lv.dinv32.as<float>());
+    TL_CHECK_LAUNCH();
+  }
+  // coarsest dense inverse
+  {
+    Level& lc = M.lv[L - 1];
+    double* A = M.cinv.as<double>();
+    TL_HIP(hipMemsetAsync(A, 0, sizeof(double) * size_t(M.cnp) * M.cnp, s));
+    const int64_t tot = lc.g.nnodes * nstencil(g.dim) * ndpn * ndpn;
+    hipLaunchKernelGGL(dense_from_stencil_kernel, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lc.g), ndpn,
+                       lc.st64.as<double>(), A, M.cnp);
+    TL_CHECK_LAUNCH();
+    if (M.cnp > M.cn) {
+      hipLaunchKernelGGL(pad_identity_kernel, dim3(1), dim3(GJB), 0, s, A, M.cn, M.cnp);
+      TL_CHECK_LAUNCH();
+    }
+    const int nbk = int(M.cnp / GJB);
+    const unsigned nt = unsigned((M.cnp + 63) / 64);
+    for (int k = 0; k < nbk; ++k) {
+      hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(GJB * GJB), 0, s, A, M.cnp, k);
+      hipLaunchKernelGGL(gj_rowpanel_kernel, dim3(unsigned((M.cnp + BS - 1) / BS)), dim3(BS), 0, s,
+                         A, M.cnp, k);
+      hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt), dim3(BS), 0, s, A, M.cnp, k);
+      hipLaunchKernelGGL(gj_colpanel_kernel, dim3(unsigned((M.cnp + BS - 1) / BS)), dim3(BS), 0, s,
+                         A, M.cnp, k);
+    }
+    TL_CHECK_LAUNCH();
+  }
+  M.ready = true;
+  return TL_OK;
+}

@@ -1,0 +1,446 @@
+// Density filter, compliance + sensitivities, OC update and loop reductions.
+//
+// Filter: filter/matrix_filter.py:101-232 builds H_ij = max(0, rr - |i-j|)
+// over the (2*ceil(rmin)-1)^ndim box (rr = ||rmin||_2/sqrt(ndim)) and applies
+// it as a CSC product (topology_optimization.py:546-558, 640).  Here H is
+// never stored: each element sums its box in increasing column order
+// (z, x, y lexicographic == increasing linear index nely*x + y + nelx*nely*z),
+// the order in which scipy's CSC product accumulates a row, with explicit
+// round-to-nearest multiply/add (no FMA contraction) so results can match the
+// CPU bit for bit.
+#include "tl_internal.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+namespace tl {
+namespace {
+
+constexpr int BS = 256;
+constexpr int kMaxBox = 13 * 13 * 13;  // rmin up to 7 in 3D
+
+__device__ inline double mul_rn(double a, double b) { return __dmul_rn(a, b); }
+__device__ inline double add_rn(double a, double b) { return __dadd_rn(a, b); }
+
+}  // namespace
+}  // namespace tl
+
+struct tl_filter {
+  int dim = 2, nelx = 0, nely = 0, nelz = 0, R = 0;
+  int64_t nel = 0;
+  double rr = 0.0;
+  hipStream_t stream = nullptr;
+  tl::DevBuf w;    // (2R+1)^dim weights, z,x,y order
+  tl::DevBuf hs;   // row sums
+  tl::DevBuf tmp;  // nel scratch
+  ~tl_filter() { w.release(); hs.release(); tmp.release(); }
+};
+
+namespace tl {
+namespace {
+
+struct FiltArgs {
+  const double* w;
+  int dim, nelx, nely, nelz, R;
+  int64_t nel;
+};
+
+// out_e = sum_{j in box(e), inside} w_ej * v_j   (v == nullptr: v_j = 1)
+__device__ inline double box_sum(const FiltArgs& a, int64_t e, const double* __restrict__ v) {
+  int x, y, z;
+  if (a.dim == 2) {
+    x = int(e / a.nely); y = int(e % a.nely); z = 0;
+  } else {
+    z = int(e / (int64_t(a.nelx) * a.nely));
+    const int64_t rest = e % (int64_t(a.nelx) * a.nely);
+    x = int(rest / a.nely); y = int(rest % a.nely);
+  }
+  const int W = 2 * a.R + 1;
+  const int zr = a.dim == 3 ? a.R : 0;
+  double acc = 0.0;
+  for (int dz = -zr; dz <= zr; ++dz) {
+    const int zz = z + dz;
+    if (zz < 0 || zz >= (a.dim == 3 ? a.nelz : 1)) continue;
+    for (int dx = -a.R; dx <= a.R; ++dx) {
+      const int xx = x + dx;
+      if (xx < 0 || xx >= a.nelx) continue;
+      const int wbase = ((dz + zr) * W + (dx + a.R)) * W;
+      const int64_t jbase = (int64_t(zz) * a.nelx + xx) * a.nely;
+      for (int dy = -a.R; dy <= a.R; ++dy) {
+        const int yy = y + dy;
+        if (yy < 0 || yy >= a.nely) continue;
+        const double wv = a.w[wbase + dy + a.R];
+        acc = add_rn(acc, v ? mul_rn(wv, v[jbase + yy]) : wv);
+      }
+    }
+  }
+  return acc;
+}
+
+__global__ void filter_hs_kernel(FiltArgs a, double* __restrict__ hs) {
+  const int64_t e = blockIdx.x * int64_t(BS) + threadIdx.x;
+  if (e < a.nel) hs[e] = box_sum(a, e, nullptr);
+}
+
+// mode 0: out = (H in)/Hs ; mode 1: in pre-divided by Hs ; mode 2: see below
+__global__ void filter_apply_kernel(FiltArgs a, const double* __restrict__ v,
+                                    const double* __restrict__ hs, const double* __restrict__ x,
+                                    double* __restrict__ out, int mode) {
+  const int64_t e = blockIdx.x * int64_t(BS) + threadIdx.x;
+  if (e >= a.nel) return;
+  const double s = box_sum(a, e, v);
+  if (mode == 0) out[e] = s / hs[e];
+  else if (mode == 1 || mode == 3) out[e] = s;
+  else out[e] = (s / hs[e]) / fmax(0.001, x[e]);
+}
+
+__global__ void div_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                           double* __restrict__ o, int64_t n) {
+  const int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x;
+  if (i < n) o[i] = a[i] / b[i];
+}
+
+__global__ void mul_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                           double* __restrict__ o, int64_t n) {
+  const int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x;
+  if (i < n) o[i] = a[i] * b[i];
+}
+
+// ---------------------------------------------------------------------------
+// compliance and sensitivities (element-centric gather)
+// ---------------------------------------------------------------------------
+template <int NDPN, bool D3>
+__global__ __launch_bounds__(BS) void compliance_kernel(
+    const double* __restrict__ u, int ncases, int64_t ndof, const double* __restrict__ xphys,
+    const double* __restrict__ KE, int nelx, int nely, int nelz, double eps, double penal,
+    double* __restrict__ dc, double* partials, unsigned* counter, double* obj_out) {
+  constexpr int NN = D3 ? 8 : 4;
+  constexpr int ND = NN * NDPN;
+  const int64_t nel = int64_t(nelx) * nely * (D3 ? nelz : 1);
+  const int64_t e = blockIdx.x * int64_t(BS) + threadIdx.x;
+  double objp = 0.0;
+  if (e < nel) {
+    int ex, ey, ez;
+    if (D3) {
+      ez = int(e / (int64_t(nelx) * nely));
+      const int64_t rest = e % (int64_t(nelx) * nely);
+      ex = int(rest / nely); ey = int(rest % nely);
+    } else {
+      ez = 0; ex = int(e / nely); ey = int(e % nely);
+    }
+    const int ny = nely + 1, nx = nelx + 1;
+    const double xe = xphys[e];
+    const double se = eps + (1.0 - eps) * pow(xe, penal);
+    const double dse = penal * (1.0 - eps) * pow(xe, penal - 1.0);
+    double dce = 0.0;
+    for (int lc = 0; lc < ncases; ++lc) {
+      const double* uc = u + int64_t(lc) * ndof;
+      double ue[ND];
+#pragma unroll
+      for (int k = 0; k < NN; ++k) {
+        const int64_t node =
+            (int64_t(ez + loc_dz(k)) * nx + ex + loc_dx(k)) * ny + ey + loc_dy(k);
+#pragma unroll
+        for (int c = 0; c < NDPN; ++c) ue[k * NDPN + c] = uc[node * NDPN + c];
+      }
+      double ce = 0.0;
+#pragma unroll
+      for (int i = 0; i < ND; ++i) {
+        double ki = 0.0;
+#pragma unroll
+        for (int j = 0; j < ND; ++j) ki += KE[i * ND + j] * ue[j];
+        ce += ue[i] * ki;
+      }
+      objp += se * ce;
+      dce -= dse * ce;
+    }
+    dc[e] = dce;
+  }
+  __shared__ double sh[BS / 64 + 2];
+  double v[1] = {objp};
+  block_sum<1>(v, sh);
+  double tot[1];
+  if (last_block_total<1>(v, partials, counter, tot, sh) && threadIdx.x == 0) *obj_out = tot[0];
+}
+
+// ---------------------------------------------------------------------------
+// OC bisection (optimizer/optimality_criterion.py:61-77), device resident
+// ---------------------------------------------------------------------------
+struct OcState {
+  double l1, l2, g, gt, move;
+  int done, steps;
+};
+
+__global__ __launch_bounds__(BS) void oc_step_kernel(const double* __restrict__ x,
+                                                     const double* __restrict__ dc,
+                                                     const double* __restrict__ dv, int64_t n,
+                                                     double* __restrict__ xnew, OcState* st,
+                                                     double* partials, unsigned* counter) {
+  if (st->done) return;
+  const double lmid = 0.5 * (st->l2 + st->l1);
+  const double move = st->move;
+  // compensated (two-sum) accumulation so that the sign decision below is
+  // the sign of the exactly rounded total
+  double s = 0.0, comp = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
+    const double xi = x[i];
+    const double cand = xi * sqrt(-dc[i] / dv[i] / lmid);
+    const double xn = fmax(0.0, fmax(xi - move, fmin(1.0, fmin(xi + move, cand))));
+    xnew[i] = xn;
+    // explicit rounding ops: a contracted FMA would break the two-sum
+    const double term = __dmul_rn(dv[i], __dsub_rn(xn, xi));
+    const double t = __dadd_rn(s, term);
+    const double bp = __dsub_rn(t, s);
+    comp = __dadd_rn(comp, __dadd_rn(__dsub_rn(s, __dsub_rn(t, bp)), __dsub_rn(term, bp)));
+    s = t;
+  }
+  __shared__ double sh[2 * (BS / 64) + 4];
+  double v[2] = {s, comp};
+  block_sum<2>(v, sh);
+  double tot[2];
+  if (last_block_total<2>(v, partials, counter, tot, sh) && threadIdx.x == 0) {
+    const double gt = st->g + (tot[0] + tot[1]);
+    st->gt = gt;
+    if (gt > 0) st->l1 = lmid;
+    else st->l2 = lmid;
+    st->steps += 1;
+    if (!((st->l2 - st->l1) / (st->l1 + st->l2) > 1e-3)) st->done = 1;
+  }
+}
+
+__global__ __launch_bounds__(BS) void mean_maxdiff_kernel(const double* __restrict__ a,
+                                                          const double* __restrict__ b, int64_t n,
+                                                          double* partials, unsigned* counter,
+                                                          double* out) {
+  double s = 0.0, m = 0.0;
+  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
+    s += a[i];
+    if (b) m = fmax(m, fabs(a[i] - b[i]));
+  }
+  m = wave_max(m);
+  __shared__ double shm[BS / 64];
+  __shared__ double sh[BS / 64 + 2];
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
+  double v[1] = {s};
+  block_sum<1>(v, sh);  // contains barriers: shm visible afterwards
+  double mb = 0.0;
+  for (int w = 0; w < BS / 64; ++w) mb = fmax(mb, shm[w]);
+  // max is order independent: publish through the second partial slot
+  double pv[2] = {v[0], mb};
+  double tot[2];
+  __shared__ double sh2[4];
+  // reuse last_block_total for the sum; the max is folded separately below
+  if (last_block_total<2>(pv, partials, counter, tot, sh2) && threadIdx.x == 0) {
+    double mx = 0.0;
+    const unsigned nb = gridDim.x;
+    for (unsigned k = 0; k < nb; ++k)
+      mx = fmax(mx, __hip_atomic_load(&partials[kMaxRedBlocks + k], __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT));
+    out[0] = tot[0] / double(n);
+    out[1] = mx;
+  }
+}
+
+unsigned nblk(int64_t n) { return unsigned(std::max<int64_t>(1, (n + BS - 1) / BS)); }
+
+struct Scratch {  // process-wide reduction scratch for the handle-free calls
+  DevBuf partials, counter, st, out;
+  int ensure() {
+    if (!partials.ptr) {
+      TL_TRY(partials.alloc(sizeof(double) * 2 * kMaxRedBlocks));
+      TL_TRY(counter.alloc(sizeof(unsigned) * 4));
+      TL_TRY(st.alloc(sizeof(OcState)));
+      TL_TRY(out.alloc(sizeof(double) * 4));
+      TL_HIP(hipMemset(counter.ptr, 0, sizeof(unsigned) * 4));
+    }
+    return TL_OK;
+  }
+};
+
+Scratch& scratch_for_device() {
+  static Scratch s[64];
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  return s[dev & 63];
+}
+
+}  // namespace
+}  // namespace tl
+
+using namespace tl;
+
+extern "C" {
+
+int tl_filter_create(int32_t ndim, int32_t nelx, int32_t nely, int32_t nelz, double rmin,
+                     void* stream, tl_filter** out) {
+  TL_ARG(out, "null argument");
+  TL_ARG(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
+  TL_ARG(nelx > 0 && nely > 0 && (ndim == 2 || nelz > 0), "element counts must be positive");
+  TL_ARG(rmin > 0.0 && std::isfinite(rmin), "rmin must be positive");
+  const int R = std::max(0, int(std::ceil(rmin)) - 1);
+  const int W = 2 * R + 1;
+  TL_ARG((ndim == 3 ? W * W * W : W * W) <= kMaxBox, "rmin too large for the filter stencil");
+  auto* f = new tl_filter();
+  f->dim = ndim; f->nelx = nelx; f->nely = nely; f->nelz = ndim == 3 ? nelz : 0; f->R = R;
+  f->nel = int64_t(nelx) * nely * (ndim == 3 ? nelz : 1);
+  f->stream = static_cast<hipStream_t>(stream);
+  // rr = ||(rmin,...,rmin)||_2 / sqrt(ndim) evaluated the way numpy's norm
+  // does it (a dot product accumulated with fused multiply-adds, then sqrt and
+  // a division), so that the filter weights match the reference bit for bit
+  double ss = rmin * rmin;
+  for (int d = 1; d < ndim; ++d) ss = std::fma(rmin, rmin, ss);
+  f->rr = std::sqrt(ss) / std::sqrt(double(ndim));
+  const int zr = ndim == 3 ? R : 0;
+  std::vector<double> w;
+  for (int dz = -zr; dz <= zr; ++dz)
+    for (int dx = -R; dx <= R; ++dx)
+      for (int dy = -R; dy <= R; ++dy) {
+        const double r = std::sqrt(double(dx * dx + dy * dy + dz * dz));
+        w.push_back(std::max(f->rr - r, 0.0));
+      }
+  int rc;
+  if ((rc = f->w.alloc(sizeof(double) * w.size())) || (rc = f->hs.alloc(sizeof(double) * f->nel)) ||
+      (rc = f->tmp.alloc(sizeof(double) * f->nel))) {
+    delete f;
+    return rc;
+  }
+  hipError_t e = hipMemcpy(f->w.ptr, w.data(), sizeof(double) * w.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    delete f;
+    set_error(std::string("tl_filter_create: ") + hipGetErrorString(e));
+    return TL_EHIP;
+  }
+  FiltArgs a{f->w.as<double>(), f->dim, f->nelx, f->nely, f->nelz, f->R, f->nel};
+  hipLaunchKernelGGL(filter_hs_kernel, dim3(nblk(f->nel)), dim3(BS), 0, f->stream, a,
+                     f->hs.as<double>());
+  e = hipGetLastError();
+  if (e != hipSuccess) {
+    delete f;
+    set_error(std::string("tl_filter_create: ") + hipGetErrorString(e));
+    return TL_EHIP;
+  }
+  *out = f;
+  return TL_OK;
+}
+
+void tl_filter_destroy(tl_filter* f) { delete f; }
+
+int tl_filter_set_stream(tl_filter* f, void* stream) {
+  TL_ARG(f, "null handle");
+  f->stream = static_cast<hipStream_t>(stream);
+  return TL_OK;
+}
+
+int tl_filter_hs(tl_filter* f, double* Hs) {
+  TL_ARG(f && Hs, "null argument");
+  TL_HIP(hipMemcpyAsync(Hs, f->hs.ptr, sizeof(double) * f->nel, hipMemcpyDeviceToDevice, f->stream));
+  return TL_OK;
+}
+
+int tl_filter_apply(tl_filter* f, const double* in, double* out, int32_t mode, const double* x) {
+  TL_ARG(f && in && out, "null argument");
+  TL_ARG(mode >= 0 && mode <= 3, "filter mode must be 0, 1, 2 or 3");
+  TL_ARG(mode != 2 || x, "mode 2 needs the design variables x");
+  TL_ARG(in != out, "in and out must not alias");
+  FiltArgs a{f->w.as<double>(), f->dim, f->nelx, f->nely, f->nelz, f->R, f->nel};
+  const unsigned nb = nblk(f->nel);
+  const double* v = in;
+  double* t = f->tmp.as<double>();
+  if (mode == 1) {
+    hipLaunchKernelGGL(div_kernel, dim3(nb), dim3(BS), 0, f->stream, in, f->hs.as<double>(), t, f->nel);
+    v = t;
+  } else if (mode == 2) {
+    hipLaunchKernelGGL(mul_kernel, dim3(nb), dim3(BS), 0, f->stream, x, in, t, f->nel);
+    v = t;
+  }
+  hipLaunchKernelGGL(filter_apply_kernel, dim3(nb), dim3(BS), 0, f->stream, a, v,
+                     f->hs.as<double>(), x, out, mode);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+int tl_compliance_sens(tl_problem* P, const double* u, int32_t ncases, const double* xPhys,
+                       double* obj, double* dc) {
+  TL_ARG(P && u && xPhys && dc && obj, "null argument");
+  TL_ARG(ncases >= 1, "ncases must be >= 1");
+  const Grid& g = P->g;
+  const unsigned nb = nblk(g.nel);
+  TL_ARG(nb <= unsigned(kMaxRedBlocks) * 64u, "grid too large");
+  double* out = P->partials.as<double>() + 3 * kMaxRedBlocks;
+  // grid-stride is not used (one element per thread); reduction slots are
+  // per block, so large grids fold through a second pass of blocks
+  const bool d3 = g.dim == 3;
+  if (nb > unsigned(kMaxRedBlocks)) {
+    set_error("compliance grid exceeds the reduction slot count");
+    return TL_EINVAL;
+  }
+#define TL_COMP(N, D)                                                                          \
+  hipLaunchKernelGGL((compliance_kernel<N, D>), dim3(nb), dim3(BS), 0, P->stream, u, ncases,  \
+                     P->ndof, xPhys, P->op.KE64.as<double>(), g.nelx, g.nely, g.nelz, P->eps, \
+                     P->penal, dc, P->partials.as<double>(), P->counter.as<unsigned>(), out)
+  if (d3) {
+    switch (P->ndpn) {
+      case 1: TL_COMP(1, true); break;
+      case 3: TL_COMP(3, true); break;
+      default: set_error("unsupported DOFs per node"); return TL_EINVAL;
+    }
+  } else {
+    switch (P->ndpn) {
+      case 1: TL_COMP(1, false); break;
+      case 2: TL_COMP(2, false); break;
+      default: set_error("unsupported DOFs per node"); return TL_EINVAL;
+    }
+  }
+#undef TL_COMP
+  TL_CHECK_LAUNCH();
+  TL_HIP(hipMemcpyAsync(obj, out, sizeof(double), hipMemcpyDeviceToHost, P->stream));
+  TL_HIP(hipStreamSynchronize(P->stream));
+  return TL_OK;
+}
+
+int tl_oc_update(int64_t n, const double* x, const double* dc, const double* dv, double g,
+                 double move, double l1, double l2, double* xnew, double* gt, void* stream) {
+  TL_ARG(x && dc && dv && xnew && gt, "null argument");
+  TL_ARG(n > 0, "n must be positive");
+  Scratch& S = scratch_for_device();
+  TL_TRY(S.ensure());
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  OcState h{l1, l2, g, g, move, 0, 0};
+  h.done = ((l2 - l1) / (l1 + l2) > 1e-3) ? 0 : 1;
+  TL_HIP(hipMemcpyAsync(S.st.ptr, &h, sizeof(OcState), hipMemcpyHostToDevice, s));
+  const unsigned nb = unsigned(std::min<int64_t>(nblk(n), 1024));
+  for (int launched = 0; launched < 400;) {
+    for (int k = 0; k < 16; ++k)
+      hipLaunchKernelGGL(oc_step_kernel, dim3(nb), dim3(BS), 0, s, x, dc, dv, n, xnew,
+                         S.st.as<OcState>(), S.partials.as<double>(), S.counter.as<unsigned>());
+    TL_CHECK_LAUNCH();
+    launched += 16;
+    TL_HIP(hipMemcpyAsync(&h, S.st.ptr, sizeof(OcState), hipMemcpyDeviceToHost, s));
+    TL_HIP(hipStreamSynchronize(s));
+    if (h.done) break;
+  }
+  TL_ARG(h.done, "OC bisection did not terminate within 400 steps");
+  *gt = h.gt;
+  return TL_OK;
+}
+
+int tl_mean_maxdiff(int64_t n, const double* a, const double* b, double* mean, double* maxdiff,
+                    void* stream) {
+  TL_ARG(a && mean && n > 0, "null argument");
+  Scratch& S = scratch_for_device();
+  TL_TRY(S.ensure());
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const unsigned nb = unsigned(std::min<int64_t>(nblk(n), 1024));
+  hipLaunchKernelGGL(mean_maxdiff_kernel, dim3(nb), dim3(BS), 0, s, a, b, n,
+                     S.partials.as<double>(), S.counter.as<unsigned>(), S.out.as<double>());
+  TL_CHECK_LAUNCH();
+  double h[2];
+  TL_HIP(hipMemcpyAsync(h, S.out.ptr, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
+  TL_HIP(hipStreamSynchronize(s));
+  *mean = h[0];
+  if (maxdiff) *maxdiff = h[1];
+  return TL_OK;
+}
+
+}  // extern "C"

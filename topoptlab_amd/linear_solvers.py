@@ -1,0 +1,33 @@
+"""Preconditioned CG with the signature of topoptlab's ``pcg``
+(topoptlab/linear_solvers.py:50-136).
+
+The reference accepts only scipy sparse ``A`` (97-98); here ``A`` is a
+device :class:`~topoptlab_amd.fem.StiffnessOperator` and ``P`` names the
+preconditioner ("gmg", "jacobi", None) or is the operator itself (GMG set up
+on it).  The stop rule is the reference's: recursive residual
+``||r||_2 < max(atol, rtol*||b||_2)`` checked before each iteration.
+Returns ``(x, info)`` with info 0 on convergence, ``maxiter-1`` otherwise.
+"""
+from __future__ import annotations
+
+import torch
+
+from .fem import StiffnessOperator
+
+
+def pcg(A: StiffnessOperator, b, P="gmg", x0=None, rtol=1e-5, atol=0.0, maxiter=100000,
+        check_every=8, **kwargs):
+    if not isinstance(A, StiffnessOperator):
+        raise TypeError("Matrix A must be a topoptlab_amd StiffnessOperator.")
+    prec = "gmg" if P is A else P
+    b = torch.as_tensor(b, dtype=torch.float64, device=A.device).reshape(-1)
+    full = b.numel() == A.ndof
+    f = b if full else A.full(b)
+    u0 = None
+    if x0 is not None:
+        x0 = torch.as_tensor(x0, dtype=torch.float64, device=A.device).reshape(-1)
+        u0 = x0 if full else A.full(x0)
+    u, info = A.solve(f, u0=u0, rtol=rtol, atol=atol, maxiter=maxiter, preconditioner=prec,
+                      check_every=check_every, raise_on_fail=False)
+    x = u if full else u[A._free_t]
+    return x, (0 if info["status"] == 0 else maxiter - 1)

@@ -1,0 +1,60 @@
+"""Synthetic inputs of the benchmark workloads (host side, numpy).
+
+The 3D cantilever benchmark uses a seeded Gaussian random density field:
+white noise on the element grid, Gaussian smoothing with a length scale of
+``sigma`` elements (periodic FFT convolution), standardisation, then
+x = lo + (1-lo)*Phi(g + b) with the shift b bisected so that mean(x) equals
+the requested mean.  Element order is the mesh's own: element id
+(iz*nelx + ix)*nely + iy in 3D, nely*ix + iy in 2D.
+"""
+from __future__ import annotations
+
+import hashlib
+
+import numpy as np
+from scipy.special import ndtr
+
+
+def gaussian_random_density(nelx, nely, nelz=None, seed=0, sigma=4.0,
+                            mean=0.3, lo=0.001):
+    """Return a float64 element density vector in [lo, 1] with the given mean."""
+    rng = np.random.default_rng(seed)
+    shape = (nelx, nely) if nelz is None else (nelz, nelx, nely)
+    g = rng.standard_normal(shape)
+    spec = np.fft.rfftn(g)
+    k2 = np.zeros(spec.shape)
+    for ax, n in enumerate(shape):
+        k = np.fft.fftfreq(n) if ax < len(shape) - 1 else np.fft.rfftfreq(n)
+        sh = [1] * len(shape)
+        sh[ax] = -1
+        k2 = k2 + (k.reshape(sh)) ** 2
+    g = np.fft.irfftn(spec * np.exp(-2.0 * np.pi ** 2 * sigma ** 2 * k2), s=shape,
+                      axes=tuple(range(len(shape))))
+    g = (g - g.mean()) / g.std()
+    a, b = -10.0, 10.0
+    for _ in range(200):
+        m = 0.5 * (a + b)
+        if (lo + (1 - lo) * ndtr(g + m)).mean() > mean:
+            b = m
+        else:
+            a = m
+    x = lo + (1 - lo) * ndtr(g + 0.5 * (a + b))
+    return np.ascontiguousarray(x.reshape(-1), dtype=np.float64)
+
+
+def field_sha256(x: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(x, dtype=np.float64).tobytes()).hexdigest()
+
+
+def seeded_loads_2d(rng, nelx, nely, ncases):
+    """Unit downward point loads at seeded boundary nodes that are not on the
+    clamped left edge (x = 0): nodes on the right, top or bottom edges."""
+    nodes = []
+    cand = ([(nelx, iy) for iy in range(nely + 1)]
+            + [(ix, 0) for ix in range(1, nelx)]
+            + [(ix, nely) for ix in range(1, nelx)])
+    pick = rng.choice(len(cand), size=ncases, replace=False)
+    for p in pick:
+        ix, iy = cand[p]
+        nodes.append((nely + 1) * ix + iy)
+    return np.array(nodes, dtype=np.int64)
