@@ -12,6 +12,7 @@
 // register moves are needed.  KE indices are compile-time constants with
 // wave-uniform addresses, so KE entries arrive as scalar operands.
 #include "tl_internal.hpp"
+#include "ke_pattern.hpp"
 
 #include <algorithm>
 #include <cmath>
@@ -114,13 +115,34 @@ struct KuArgs {
   double omega;
   void* y;              // output (TO)
   double* partials;
-  unsigned* counter;
   double* dot_out;
   PcgState* st;
   int fin;
   const int* done;
   int nelx, nely, nelz, nx, ny, nz, plane, zchunk;
+  // group values of a pattern-matched element matrix (see ke_pattern.hpp)
+  double kv64[8];
+  float kv32[8];
 };
+
+// Compile-time (+-)(group+1) pattern of the unit-cell element matrix.
+template <bool D3, int NDPN>
+__host__ __device__ constexpr int ke_pat(int i) {
+  if constexpr (D3 && NDPN == 3) return kPat_h8_elast(i);
+  else if constexpr (D3 && NDPN == 1) return kPat_h8_poisson(i);
+  else if constexpr (!D3 && NDPN == 2) return kPat_q4_elast(i);
+  else if constexpr (!D3 && NDPN == 1) return kPat_q4_poisson(i);
+  else return 0;
+}
+
+__device__ inline float fma_t(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ inline double fma_t(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+template <typename T>
+__device__ inline T kv_at(const KuArgs& a, int g) {
+  if constexpr (sizeof(T) == 8) return T(a.kv64[g]);
+  else return T(a.kv32[g]);
+}
 
 template <typename T>
 __device__ inline T ldval(const void* p, int64_t i) {
@@ -160,7 +182,7 @@ __device__ inline T stage_val(const KuArgs& a, int64_t g, bool fusep) {
 
 // Contribution of the 4 (2D) / 8 (3D) elements around a node.
 // U: neighbour values [3 slots][9][NDPN]; slot index of relative plane rz.
-template <int NDPN, typename T, int SM, int SC, int SP, bool D3>
+template <int NDPN, typename T, int SM, int SC, int SP, bool D3, bool FAST>
 __device__ inline void node_sum(const KuArgs& a, const T (&U)[3][9][NDPN],
                                 const T (&S)[3][4], int sdn, int sup, T (&acc)[NDPN]) {
   constexpr int ND = (D3 ? 8 : 4) * NDPN;
@@ -195,11 +217,23 @@ __device__ inline void node_sum(const KuArgs& a, const T (&U)[3][9][NDPN],
 #pragma unroll
           for (int d = 0; d < NDPN; ++d)
 #pragma unroll
-            for (int c = 0; c < NDPN; ++c)
-              t[d] += KE[(la * NDPN + d) * ND + lb * NDPN + c] * U[slot][nb][c];
+            for (int c = 0; c < NDPN; ++c) {
+              const int e = (la * NDPN + d) * ND + lb * NDPN + c;
+              // explicit fused multiply-adds: each term stays one FMA (the
+              // compiler may otherwise share kv*U products between elements
+              // as separate multiplies held in registers)
+              if constexpr (FAST) {
+                // group value resident in SGPRs; the sign folds into the FMA
+                const int pg = ke_pat<D3, NDPN>(e);
+                const T kv = kv_at<T>(a, (pg > 0 ? pg : -pg) - 1);
+                t[d] = fma_t(pg > 0 ? kv : -kv, U[slot][nb][c], t[d]);
+              } else {
+                t[d] = fma_t(T(KE[e]), U[slot][nb][c], t[d]);
+              }
+            }
         }
 #pragma unroll
-        for (int d = 0; d < NDPN; ++d) acc[d] += s * t[d];
+        for (int d = 0; d < NDPN; ++d) acc[d] = fma_t(s, t[d], acc[d]);
       }
     }
   }
@@ -242,8 +276,8 @@ __device__ inline void load_dens(const KuArgs& a, int ix, int iy, int ez, T (&S)
       S[(ox + 1) * 2 + (oy + 1)] = sp[(rowz + ix + ox + 1) * (a.nely + 2) + iy + oy + 1];
 }
 
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int SA,
-          int SB, int SC>
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST,
+          int SA, int SB, int SC>
 __device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t j0,
                               int64_t j, int ix, int iy, bool active, T (&U)[3][9][NDPN],
                               T (&S)[3][4], double& dot) {
@@ -254,7 +288,7 @@ __device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t
   __syncthreads();
   read_nbrs<NDPN, T, SC>(a, buf, U);
   T acc[NDPN];
-  node_sum<NDPN, T, SA, SB, SC, true>(a, U, S, SA, SB, acc);
+  node_sum<NDPN, T, SA, SB, SC, true, FAST>(a, U, S, SA, SB, acc);
   if (active) {
     const int64_t node = int64_t(iz) * a.plane + j;
     T uc[NDPN];
@@ -268,7 +302,7 @@ __device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t
   }
 }
 
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP>
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
 __global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
   if (a.done && *a.done) return;
   extern __shared__ double lds_raw[];
@@ -295,26 +329,24 @@ __global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
   __syncthreads();
   // planes of node iz use LDS buffer (iz & 1); rotating slots (0,1,2)
   for (int iz = z0; iz < z1; iz += 3) {
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, 0, 1, 2>(a, lds0, lds1, iz, j0, j, ix, iy, inplane, U, S, dot);
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 0, 1, 2>(a, lds0, lds1, iz, j0, j, ix, iy, inplane, U, S, dot);
     __syncthreads();
     if (iz + 1 >= z1) break;
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, 1, 2, 0>(a, lds0, lds1, iz + 1, j0, j, ix, iy, inplane, U, S, dot);
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 1, 2, 0>(a, lds0, lds1, iz + 1, j0, j, ix, iy, inplane, U, S, dot);
     __syncthreads();
     if (iz + 2 >= z1) break;
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, 2, 0, 1>(a, lds0, lds1, iz + 2, j0, j, ix, iy, inplane, U, S, dot);
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 2, 0, 1>(a, lds0, lds1, iz + 2, j0, j, ix, iy, inplane, U, S, dot);
     __syncthreads();
   }
   if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
     __shared__ double sh[BS / 64 + 2];
     double v[1] = {dot};
     block_sum<1>(v, sh);
-    double tot[1];
-    if (last_block_total<1>(v, a.partials, a.counter, tot, sh) && threadIdx.x == 0)
-      finalize(a.fin, a.st, tot[0], a.dot_out);
+    store_partials<1>(v, a.partials);
   }
 }
 
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP>
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
 __global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
   if (a.done && *a.done) return;
   extern __shared__ double lds_raw[];
@@ -339,7 +371,7 @@ __global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
   __syncthreads();
   read_nbrs<NDPN, T, 1>(a, lds, U);
   T acc[NDPN];
-  node_sum<NDPN, T, 0, 1, 2, false>(a, U, S, 0, 1, acc);
+  node_sum<NDPN, T, 0, 1, 2, false, FAST>(a, U, S, 0, 1, acc);
   double dot = 0.0;
   if (active) {
     T uc[NDPN];
@@ -355,14 +387,15 @@ __global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
     __shared__ double sh[BS / 64 + 2];
     double v[1] = {dot};
     block_sum<1>(v, sh);
-    double tot[1];
-    if (last_block_total<1>(v, a.partials, a.counter, tot, sh) && threadIdx.x == 0)
-      finalize(a.fin, a.st, tot[0], a.dot_out);
+    store_partials<1>(v, a.partials);
   }
 }
 
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP>
-static int launch_ku(const tl_problem* P, KuArgs a) {
+static int launch_fold(tl_problem* P, unsigned nb, int fin, double* out, const int* done);
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
+static int launch_ku_impl(const tl_problem* P, KuArgs a) {
+
   const Grid& g = P->g;
   a.nelx = g.nelx; a.nely = g.nely; a.nelz = g.nelz;
   a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.plane = g.nx * g.ny;
@@ -378,19 +411,33 @@ static int launch_ku(const tl_problem* P, KuArgs a) {
       set_error("fine operator grid exceeds the reduction slot count");
       return TL_EINVAL;
     }
-    hipLaunchKernelGGL((ku3d_kernel<NDPN, T, MODE, TB, TO, FUSEP>), grid, dim3(BS),
+    hipLaunchKernelGGL((ku3d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), grid, dim3(BS),
                        size_t(2) * L * NDPN * sizeof(T), P->stream, a);
+    if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
+      TL_CHECK_LAUNCH();
+      return launch_fold(const_cast<tl_problem*>(P), grid.x * grid.y, a.fin, a.dot_out, a.done);
+    }
   } else {
     a.zchunk = 1;
     if (nbx > unsigned(kMaxRedBlocks) && (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)) {
       set_error("fine operator grid exceeds the reduction slot count");
       return TL_EINVAL;
     }
-    hipLaunchKernelGGL((ku2d_kernel<NDPN, T, MODE, TB, TO, FUSEP>), dim3(nbx), dim3(BS),
+    hipLaunchKernelGGL((ku2d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), dim3(nbx), dim3(BS),
                        size_t(L) * NDPN * sizeof(T), P->stream, a);
+    if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
+      TL_CHECK_LAUNCH();
+      return launch_fold(const_cast<tl_problem*>(P), nbx, a.fin, a.dot_out, a.done);
+    }
   }
   TL_CHECK_LAUNCH();
   return TL_OK;
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP>
+static int launch_ku(const tl_problem* P, KuArgs a) {
+  if (P->op.fast_ke) return launch_ku_impl<NDPN, T, MODE, TB, TO, FUSEP, true>(P, a);
+  return launch_ku_impl<NDPN, T, MODE, TB, TO, FUSEP, false>(P, a);
 }
 
 template <int NDPN, typename T>
@@ -435,11 +482,14 @@ int fine_apply_ex(tl_problem* P, const void* u, void* y, int prec, int mode, con
   a.pin = zfuse_pin; a.pout = pout; a.beta = beta;
   a.mask = P->op.mask.as<uint8_t>();
   a.partials = P->partials.as<double>();
-  a.counter = P->counter.as<unsigned>();
   a.dot_out = dot_out;
   a.st = P->state.as<PcgState>();
   a.fin = fin;
   a.done = done;
+  for (int g = 0; g < 8; ++g) {
+    a.kv64[g] = P->op.kv[g];
+    a.kv32[g] = float(P->op.kv[g]);
+  }
   const bool fusep = zfuse_pin != nullptr;
   ProfEvent ev{};
   const bool prof = P->prof_on;
@@ -549,32 +599,42 @@ int fine_diag(tl_problem* P) {
 // vector kernels of the PCG loop
 // ----------------------------------------------------------------------------
 template <int NV>
-__device__ inline void reduce_finish(double (&v)[NV], double* partials, unsigned* counter,
-                                     int fin, PcgState* st, double* out) {
+__device__ inline void reduce_store(double (&v)[NV], double* partials) {
   __shared__ double sh[NV * (BS / 64) + 4];
   block_sum<NV>(v, sh);
-  double tot[NV];
-  if (last_block_total<NV>(v, partials, counter, tot, sh) && threadIdx.x == 0)
-    finalize(fin, st, tot[0], out);
+  store_partials<NV>(v, partials);
+}
+
+// One block: fold the partials of the preceding kernel (nb blocks) and apply
+// the scalar update `fin`.  Skipped, like the kernel it follows, once the
+// solve is done.
+__global__ __launch_bounds__(BS) void fold_kernel(const double* __restrict__ partials, unsigned nb,
+                                                  int fin, PcgState* st, double* out,
+                                                  const int* done) {
+  if (done && *done) return;
+  __shared__ double sh[BS / 64 + 2];
+  double tot[1];
+  fold_partials<1>(partials, nb, tot, sh);
+  if (threadIdx.x == 0) finalize(fin, st, tot[0], out);
 }
 
 static unsigned red_blocks(int64_t n) {
+  // enough workgroups to keep HBM busy (each thread streams ~5 elements at
+  // the 5.4M-DOF size), bounded by the reduction slot count
   int64_t nb = (n + BS - 1) / BS;
-  nb = std::min<int64_t>(nb, 1024);
+  nb = std::min<int64_t>(nb, kMaxRedBlocks);
   return unsigned(std::max<int64_t>(nb, 1));
 }
 
 // out = a.b (fin selects where the total goes)
 __global__ __launch_bounds__(BS) void dot_kernel(const double* __restrict__ a,
                                                  const double* __restrict__ b, int64_t n,
-                                                 double* partials, unsigned* counter, int fin,
-                                                 PcgState* st, double* out, const int* done) {
-  if (done && *done) return;
+                                                 double* partials) {
   double s = 0.0;
   for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS)
     s += a[i] * b[i];
   double v[1] = {s};
-  reduce_finish<1>(v, partials, counter, fin, st, out);
+  reduce_store<1>(v, partials);
 }
 
 // r = b - y (masked by y's zeros already); rr
@@ -582,8 +642,7 @@ __global__ __launch_bounds__(BS) void resid_kernel(const double* __restrict__ b,
                                                    const double* __restrict__ y,
                                                    const uint8_t* __restrict__ mask, int ndpn,
                                                    double* __restrict__ r, int64_t n,
-                                                   double* partials, unsigned* counter, int fin,
-                                                   PcgState* st, double* out) {
+                                                   double* partials) {
   double s = 0.0;
   for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
     const bool fixed = (mask[i / ndpn] >> (i % ndpn)) & 1;
@@ -592,15 +651,15 @@ __global__ __launch_bounds__(BS) void resid_kernel(const double* __restrict__ b,
     s += v * v;
   }
   double v[1] = {s};
-  reduce_finish<1>(v, partials, counter, fin, st, out);
+  reduce_store<1>(v, partials);
 }
 
 // x += alpha p ; r -= alpha q ; rr
 __global__ __launch_bounds__(BS) void update_kernel(double* __restrict__ x, double* __restrict__ r,
                                                     const double* __restrict__ p,
                                                     const double* __restrict__ q, int64_t n,
-                                                    double* partials, unsigned* counter,
-                                                    PcgState* st, const int* done) {
+                                                    double* partials, const PcgState* st,
+                                                    const int* done) {
   if (*done) return;
   const double alpha = st->alpha;
   double s = 0.0;
@@ -611,15 +670,14 @@ __global__ __launch_bounds__(BS) void update_kernel(double* __restrict__ x, doub
     s += rv * rv;
   }
   double v[1] = {s};
-  reduce_finish<1>(v, partials, counter, FIN_RR, st, nullptr);
+  reduce_store<1>(v, partials);
 }
 
 // z = dinv * r (or z = r when dinv == nullptr); rho = r.z
 __global__ __launch_bounds__(BS) void jacobi_prec_kernel(const double* __restrict__ r,
                                                          const double* __restrict__ dinv,
                                                          double* __restrict__ z, int64_t n,
-                                                         double* partials, unsigned* counter,
-                                                         PcgState* st, const int* done) {
+                                                         double* partials, const int* done) {
   if (*done) return;
   double s = 0.0;
   for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
@@ -628,7 +686,7 @@ __global__ __launch_bounds__(BS) void jacobi_prec_kernel(const double* __restric
     s += r[i] * zv;
   }
   double v[1] = {s};
-  reduce_finish<1>(v, partials, counter, FIN_RHO, st, nullptr);
+  reduce_store<1>(v, partials);
 }
 
 // masked copy (fixed DOFs -> 0)
@@ -667,10 +725,9 @@ static unsigned grid_for(int64_t n) {
   return unsigned(std::max<int64_t>(1, std::min<int64_t>((n + BS - 1) / BS, 8192)));
 }
 
-int reduce_dot(tl_problem* P, const double* a, const double* b, int64_t n, double* out_dev) {
-  hipLaunchKernelGGL(dot_kernel, dim3(red_blocks(n)), dim3(BS), 0, P->stream, a, b, n,
-                     P->partials.as<double>(), P->counter.as<unsigned>(), FIN_STORE,
-                     P->state.as<PcgState>(), out_dev, nullptr);
+static int launch_fold(tl_problem* P, unsigned nb, int fin, double* out, const int* done) {
+  hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(BS), 0, P->stream, P->partials.as<double>(), nb,
+                     fin, P->state.as<PcgState>(), out, done);
   TL_CHECK_LAUNCH();
   return TL_OK;
 }
@@ -693,8 +750,35 @@ tl_problem::~tl_problem() {
   }
   mg.G1.release(); mg.W.release(); mg.cinv.release();
   r.release(); z.release(); p.release(); p2.release(); q.release(); t.release();
-  partials.release(); counter.release(); state.release();
+  partials.release(); state.release();
 }
+
+namespace tl {
+// Use the pattern path when every KE entry equals (+-) its group's first
+// entry to 1e-13 of max|KE|; the kernels then use the group values.
+static void detect_ke_pattern(tl_problem* P) {
+  P->op.fast_ke = false;
+  const int nd = P->nd;
+  const bool d3 = P->g.dim == 3;
+  int (*pat)(int) = nullptr;
+  const int* first = nullptr;
+  int ng = 0;
+  if (d3 && P->ndpn == 3) { pat = [](int i) { return kPat_h8_elast(i); }; first = kFirst_h8_elast; ng = kGroups_h8_elast; }
+  else if (d3 && P->ndpn == 1) { pat = [](int i) { return kPat_h8_poisson(i); }; first = kFirst_h8_poisson; ng = kGroups_h8_poisson; }
+  else if (!d3 && P->ndpn == 2) { pat = [](int i) { return kPat_q4_elast(i); }; first = kFirst_q4_elast; ng = kGroups_q4_elast; }
+  else if (!d3 && P->ndpn == 1) { pat = [](int i) { return kPat_q4_poisson(i); }; first = kFirst_q4_poisson; ng = kGroups_q4_poisson; }
+  if (!pat || ng > 8) return;
+  double mx = 0.0;
+  for (double v : P->KEh) mx = std::max(mx, std::fabs(v));
+  for (int g = 0; g < ng; ++g) P->op.kv[g] = P->KEh[first[g]];
+  for (int i = 0; i < nd * nd; ++i) {
+    const int pg = pat(i);
+    const double v = (pg > 0 ? 1.0 : -1.0) * P->op.kv[(pg > 0 ? pg : -pg) - 1];
+    if (std::fabs(v - P->KEh[i]) > 1e-13 * mx) return;
+  }
+  P->op.fast_ke = true;
+}
+}  // namespace tl
 
 using namespace tl;
 
@@ -736,6 +820,7 @@ int tl_create(const tl_grid_desc* d, tl_problem** out) {
     mask[f / d->ndpn] |= uint8_t(1u << (f % d->ndpn));
   }
   P->fixed_h.assign(d->fixed, d->fixed + d->nfixed);
+  detect_ke_pattern(P);
   std::vector<float> ke32(nd * nd);
   for (int i = 0; i < nd * nd; ++i) ke32[i] = float(P->KEh[i]);
   auto fail = [&](int rc) { delete P; return rc; };
@@ -748,14 +833,12 @@ int tl_create(const tl_grid_desc* d, tl_problem** out) {
       (rc = P->op.dinv64.alloc(sizeof(double) * P->ndof)) ||
       (rc = P->op.dinv32.alloc(sizeof(float) * P->ndof)) ||
       (rc = P->partials.alloc(sizeof(double) * 4 * kMaxRedBlocks)) ||
-      (rc = P->counter.alloc(sizeof(unsigned) * 4)) ||
       (rc = P->state.alloc(sizeof(PcgState))))
     return fail(rc);
   hipError_t e;
   if ((e = hipMemcpy(P->op.KE64.ptr, P->KEh.data(), sizeof(double) * nd * nd, hipMemcpyHostToDevice)) ||
       (e = hipMemcpy(P->op.KE32.ptr, ke32.data(), sizeof(float) * nd * nd, hipMemcpyHostToDevice)) ||
       (e = hipMemcpy(P->op.mask.ptr, mask.data(), P->g.nnodes, hipMemcpyHostToDevice)) ||
-      (e = hipMemset(P->counter.ptr, 0, sizeof(unsigned) * 4)) ||
       (e = hipMemset(P->state.ptr, 0, sizeof(PcgState))) ||
       (e = hipMemset(P->op.sp64.ptr, 0, sizeof(double) * P->g.npad())) ||
       (e = hipMemset(P->op.sp32.ptr, 0, sizeof(float) * P->g.npad()))) {
@@ -870,16 +953,15 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
   PcgState* st = P->state.as<PcgState>();
   const int* done = &st->done;
   double* partials = P->partials.as<double>();
-  unsigned* counter = P->counter.as<unsigned>();
   const uint8_t* mask = P->op.mask.as<uint8_t>();
   const unsigned nb = grid_for(n), rb = red_blocks(n);
   hipStream_t s = P->stream;
   // b (masked) -> t ; bb
   hipLaunchKernelGGL(masked_copy_kernel, dim3(nb), dim3(BS), 0, s, f, t, mask, P->ndpn, n);
   TL_CHECK_LAUNCH();
-  hipLaunchKernelGGL(dot_kernel, dim3(rb), dim3(BS), 0, s, t, t, n, partials, counter, FIN_BB,
-                     st, nullptr, nullptr);
+  hipLaunchKernelGGL(dot_kernel, dim3(rb), dim3(BS), 0, s, t, t, n, partials);
   TL_CHECK_LAUNCH();
+  TL_TRY(launch_fold(P, rb, FIN_BB, nullptr, nullptr));
   hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(1), 0, s, st, o->rtol, o->atol, o->maxiter);
   TL_CHECK_LAUNCH();
   // x0: masked copy of u, or zero
@@ -889,14 +971,15 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
     TL_TRY(fine_apply_ex(P, u, q, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0, nullptr,
                          nullptr, nullptr, nullptr, FIN_STORE, nullptr, true));
     hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn, r, n,
-                       partials, counter, FIN_RR0, st, nullptr);
+                       partials);
   } else {
     TL_HIP(hipMemsetAsync(u, 0, sizeof(double) * n, s));
     TL_HIP(hipMemsetAsync(q, 0, sizeof(double) * n, s));
     hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn, r, n,
-                       partials, counter, FIN_RR0, st, nullptr);
+                       partials);
   }
   TL_CHECK_LAUNCH();
+  TL_TRY(launch_fold(P, rb, FIN_RR0, nullptr, nullptr));
   // search directions ping-pong between two buffers: the fused product
   // reads p_k (halo included) while it writes p_{k+1}
   double* pp[2] = {p, P->p2.as<double>()};
@@ -913,8 +996,9 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
       } else {
         hipLaunchKernelGGL(jacobi_prec_kernel, dim3(rb), dim3(BS), 0, s, r,
                            o->precond == TL_PRECOND_JACOBI ? P->op.dinv64.as<double>() : nullptr,
-                           z, n, partials, counter, st, done);
+                           z, n, partials, done);
         TL_CHECK_LAUNCH();
+        TL_TRY(launch_fold(P, rb, FIN_RHO, nullptr, done));
       }
       // p = z + beta p ; q = K p ; pq ; alpha
       double* pin = pp[kk & 1];
@@ -924,8 +1008,9 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
                            pout, &st->beta, nullptr, FIN_PQ, done, false));
       // x += alpha p ; r -= alpha q ; rr ; done
       hipLaunchKernelGGL(update_kernel, dim3(rb), dim3(BS), 0, s, u, r, pout, q, n, partials,
-                         counter, st, done);
+                         st, done);
       TL_CHECK_LAUNCH();
+      TL_TRY(launch_fold(P, rb, FIN_RR, nullptr, done));
     }
     launched += chunk;
     TL_HIP(hipMemcpyAsync(&hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
@@ -938,8 +1023,9 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
                        nullptr, nullptr, nullptr, FIN_STORE, nullptr, true));
   double* scal = P->partials.as<double>() + 3 * kMaxRedBlocks;  // scratch slot for the total
   hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn,
-                     static_cast<double*>(nullptr), n, partials, counter, FIN_STORE, st, scal);
+                     static_cast<double*>(nullptr), n, partials);
   TL_CHECK_LAUNCH();
+  TL_TRY(launch_fold(P, rb, FIN_STORE, scal, nullptr));
   double tr2 = 0.0;
   TL_HIP(hipMemcpyAsync(&tr2, scal, sizeof(double), hipMemcpyDeviceToHost, s));
   TL_HIP(hipMemcpyAsync(&hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));

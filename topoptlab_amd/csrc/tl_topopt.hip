@@ -114,13 +114,13 @@ template <int NDPN, bool D3>
 __global__ __launch_bounds__(BS) void compliance_kernel(
     const double* __restrict__ u, int ncases, int64_t ndof, const double* __restrict__ xphys,
     const double* __restrict__ KE, int nelx, int nely, int nelz, double eps, double penal,
-    double* __restrict__ dc, double* partials, unsigned* counter, double* obj_out) {
+    double* __restrict__ dc, double* partials) {
   constexpr int NN = D3 ? 8 : 4;
   constexpr int ND = NN * NDPN;
   const int64_t nel = int64_t(nelx) * nely * (D3 ? nelz : 1);
-  const int64_t e = blockIdx.x * int64_t(BS) + threadIdx.x;
   double objp = 0.0;
-  if (e < nel) {
+  for (int64_t e = blockIdx.x * int64_t(BS) + threadIdx.x; e < nel;
+       e += int64_t(gridDim.x) * BS) {
     int ex, ey, ez;
     if (D3) {
       ez = int(e / (int64_t(nelx) * nely));
@@ -160,8 +160,16 @@ __global__ __launch_bounds__(BS) void compliance_kernel(
   __shared__ double sh[BS / 64 + 2];
   double v[1] = {objp};
   block_sum<1>(v, sh);
+  store_partials<1>(v, partials);
+}
+
+// one block: total of the preceding kernel's nb partials -> out[0]
+__global__ __launch_bounds__(BS) void fold_sum_kernel(const double* __restrict__ partials,
+                                                      unsigned nb, double* out) {
+  __shared__ double sh[BS / 64 + 2];
   double tot[1];
-  if (last_block_total<1>(v, partials, counter, tot, sh) && threadIdx.x == 0) *obj_out = tot[0];
+  fold_partials<1>(partials, nb, tot, sh);
+  if (threadIdx.x == 0) out[0] = tot[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -175,8 +183,8 @@ struct OcState {
 __global__ __launch_bounds__(BS) void oc_step_kernel(const double* __restrict__ x,
                                                      const double* __restrict__ dc,
                                                      const double* __restrict__ dv, int64_t n,
-                                                     double* __restrict__ xnew, OcState* st,
-                                                     double* partials, unsigned* counter) {
+                                                     double* __restrict__ xnew, const OcState* st,
+                                                     double* partials) {
   if (st->done) return;
   const double lmid = 0.5 * (st->l2 + st->l1);
   const double move = st->move;
@@ -198,8 +206,18 @@ __global__ __launch_bounds__(BS) void oc_step_kernel(const double* __restrict__ 
   __shared__ double sh[2 * (BS / 64) + 4];
   double v[2] = {s, comp};
   block_sum<2>(v, sh);
+  store_partials<2>(v, partials);
+}
+
+// one block: gt = g + sum dv*(xnew - x); bisect (optimality_criterion.py:72-77)
+__global__ __launch_bounds__(BS) void oc_fold_kernel(const double* __restrict__ partials,
+                                                     unsigned nb, OcState* st) {
+  if (st->done) return;
+  __shared__ double sh[2 * (BS / 64) + 4];
   double tot[2];
-  if (last_block_total<2>(v, partials, counter, tot, sh) && threadIdx.x == 0) {
+  fold_partials<2>(partials, nb, tot, sh);
+  if (threadIdx.x == 0) {
+    const double lmid = 0.5 * (st->l2 + st->l1);
     const double gt = st->g + (tot[0] + tot[1]);
     st->gt = gt;
     if (gt > 0) st->l1 = lmid;
@@ -211,8 +229,7 @@ __global__ __launch_bounds__(BS) void oc_step_kernel(const double* __restrict__ 
 
 __global__ __launch_bounds__(BS) void mean_maxdiff_kernel(const double* __restrict__ a,
                                                           const double* __restrict__ b, int64_t n,
-                                                          double* partials, unsigned* counter,
-                                                          double* out) {
+                                                          double* partials) {
   double s = 0.0, m = 0.0;
   for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
     s += a[i];
@@ -223,36 +240,43 @@ __global__ __launch_bounds__(BS) void mean_maxdiff_kernel(const double* __restri
   __shared__ double sh[BS / 64 + 2];
   if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = m;
   double v[1] = {s};
-  block_sum<1>(v, sh);  // contains barriers: shm visible afterwards
+  block_sum<1>(v, sh);  // its barriers also publish shm
   double mb = 0.0;
   for (int w = 0; w < BS / 64; ++w) mb = fmax(mb, shm[w]);
-  // max is order independent: publish through the second partial slot
   double pv[2] = {v[0], mb};
-  double tot[2];
-  __shared__ double sh2[4];
-  // reuse last_block_total for the sum; the max is folded separately below
-  if (last_block_total<2>(pv, partials, counter, tot, sh2) && threadIdx.x == 0) {
-    double mx = 0.0;
-    const unsigned nb = gridDim.x;
-    for (unsigned k = 0; k < nb; ++k)
-      mx = fmax(mx, __hip_atomic_load(&partials[kMaxRedBlocks + k], __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT));
+  store_partials<2>(pv, partials);
+}
+
+// one block: out[0] = mean, out[1] = max of the block maxima
+__global__ __launch_bounds__(BS) void mean_maxdiff_fold_kernel(const double* __restrict__ partials,
+                                                               unsigned nb, int64_t n,
+                                                               double* out) {
+  __shared__ double sh[BS / 64 + 2];
+  double tot[1];
+  fold_partials<1>(partials, nb, tot, sh);
+  double mx = 0.0;
+  for (unsigned b = threadIdx.x; b < nb; b += BS) mx = fmax(mx, partials[kMaxRedBlocks + b]);
+  mx = wave_max(mx);
+  __shared__ double shm[BS / 64];
+  if ((threadIdx.x & 63) == 0) shm[threadIdx.x >> 6] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m2 = 0.0;
+    for (int w = 0; w < BS / 64; ++w) m2 = fmax(m2, shm[w]);
     out[0] = tot[0] / double(n);
-    out[1] = mx;
+    out[1] = m2;
   }
 }
 
 unsigned nblk(int64_t n) { return unsigned(std::max<int64_t>(1, (n + BS - 1) / BS)); }
 
 struct Scratch {  // process-wide reduction scratch for the handle-free calls
-  DevBuf partials, counter, st, out;
+  DevBuf partials, st, out;
   int ensure() {
     if (!partials.ptr) {
       TL_TRY(partials.alloc(sizeof(double) * 2 * kMaxRedBlocks));
-      TL_TRY(counter.alloc(sizeof(unsigned) * 4));
       TL_TRY(st.alloc(sizeof(OcState)));
       TL_TRY(out.alloc(sizeof(double) * 4));
-      TL_HIP(hipMemset(counter.ptr, 0, sizeof(unsigned) * 4));
     }
     return TL_OK;
   }
@@ -365,20 +389,13 @@ int tl_compliance_sens(tl_problem* P, const double* u, int32_t ncases, const dou
   TL_ARG(P && u && xPhys && dc && obj, "null argument");
   TL_ARG(ncases >= 1, "ncases must be >= 1");
   const Grid& g = P->g;
-  const unsigned nb = nblk(g.nel);
-  TL_ARG(nb <= unsigned(kMaxRedBlocks) * 64u, "grid too large");
+  const unsigned nb = unsigned(std::min<int64_t>(nblk(g.nel), kMaxRedBlocks));
   double* out = P->partials.as<double>() + 3 * kMaxRedBlocks;
-  // grid-stride is not used (one element per thread); reduction slots are
-  // per block, so large grids fold through a second pass of blocks
   const bool d3 = g.dim == 3;
-  if (nb > unsigned(kMaxRedBlocks)) {
-    set_error("compliance grid exceeds the reduction slot count");
-    return TL_EINVAL;
-  }
 #define TL_COMP(N, D)                                                                          \
   hipLaunchKernelGGL((compliance_kernel<N, D>), dim3(nb), dim3(BS), 0, P->stream, u, ncases,  \
                      P->ndof, xPhys, P->op.KE64.as<double>(), g.nelx, g.nely, g.nelz, P->eps, \
-                     P->penal, dc, P->partials.as<double>(), P->counter.as<unsigned>(), out)
+                     P->penal, dc, P->partials.as<double>())
   if (d3) {
     switch (P->ndpn) {
       case 1: TL_COMP(1, true); break;
@@ -393,6 +410,9 @@ int tl_compliance_sens(tl_problem* P, const double* u, int32_t ncases, const dou
     }
   }
 #undef TL_COMP
+  TL_CHECK_LAUNCH();
+  hipLaunchKernelGGL(fold_sum_kernel, dim3(1), dim3(BS), 0, P->stream, P->partials.as<double>(),
+                     nb, out);
   TL_CHECK_LAUNCH();
   TL_HIP(hipMemcpyAsync(obj, out, sizeof(double), hipMemcpyDeviceToHost, P->stream));
   TL_HIP(hipStreamSynchronize(P->stream));
@@ -411,9 +431,12 @@ int tl_oc_update(int64_t n, const double* x, const double* dc, const double* dv,
   TL_HIP(hipMemcpyAsync(S.st.ptr, &h, sizeof(OcState), hipMemcpyHostToDevice, s));
   const unsigned nb = unsigned(std::min<int64_t>(nblk(n), 1024));
   for (int launched = 0; launched < 400;) {
-    for (int k = 0; k < 16; ++k)
+    for (int k = 0; k < 16; ++k) {
       hipLaunchKernelGGL(oc_step_kernel, dim3(nb), dim3(BS), 0, s, x, dc, dv, n, xnew,
-                         S.st.as<OcState>(), S.partials.as<double>(), S.counter.as<unsigned>());
+                         S.st.as<OcState>(), S.partials.as<double>());
+      hipLaunchKernelGGL(oc_fold_kernel, dim3(1), dim3(BS), 0, s, S.partials.as<double>(), nb,
+                         S.st.as<OcState>());
+    }
     TL_CHECK_LAUNCH();
     launched += 16;
     TL_HIP(hipMemcpyAsync(&h, S.st.ptr, sizeof(OcState), hipMemcpyDeviceToHost, s));
@@ -433,7 +456,9 @@ int tl_mean_maxdiff(int64_t n, const double* a, const double* b, double* mean, d
   hipStream_t s = static_cast<hipStream_t>(stream);
   const unsigned nb = unsigned(std::min<int64_t>(nblk(n), 1024));
   hipLaunchKernelGGL(mean_maxdiff_kernel, dim3(nb), dim3(BS), 0, s, a, b, n,
-                     S.partials.as<double>(), S.counter.as<unsigned>(), S.out.as<double>());
+                     S.partials.as<double>());
+  hipLaunchKernelGGL(mean_maxdiff_fold_kernel, dim3(1), dim3(BS), 0, s, S.partials.as<double>(),
+                     nb, n, S.out.as<double>());
   TL_CHECK_LAUNCH();
   double h[2];
   TL_HIP(hipMemcpyAsync(h, S.out.ptr, sizeof(double) * 2, hipMemcpyDeviceToHost, s));
