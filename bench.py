@@ -65,10 +65,11 @@ def solve_traffic_bytes(dims, nlevels, iters, vprec_bytes=4):
     return iters * (fine_pcg + fine_v + coarse)
 
 
-def cpu_baseline(sample=(48, 24, 24), nlevels=4):
+def cpu_baseline(sample=(96, 48, 48), nlevels=4):
     """The oracle (numpy/scipy, FP64 GMG-PCG with CSR operators, Galerkin
     hierarchy built once per solve) on a bounded sample of the same workload:
-    same boundary conditions and density generator at 48x24x24."""
+    same boundary conditions and density generator at 96x48x48 (coarsest grid
+    12x6x6, as on the GPU at full size)."""
     from oracle import fem, mg
     from topoptlab_amd.workloads import gaussian_random_density
     nelx, nely, nelz = sample
@@ -94,7 +95,19 @@ def main():
     ap.add_argument("--vcycle", default="fp32", choices=["fp32", "fp64"])
     ap.add_argument("--nelx", type=int, default=192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain device copy of 256 MiB (5 launches) so that "
+                         "FETCH_SIZE/WRITE_SIZE counters can be scaled to bytes")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+        a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"copy calibration: 5 x {a.numel() * 8} bytes read and written")
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the 192x96x96 solve runs on a single MI355X); use --gpus 1", file=sys.stderr)
@@ -142,37 +155,44 @@ def main():
     ms = 1e3 * t / args.steps
     import ctypes as C
     prof = {}
-    for kind in (0, 1):
+    for kind in (0, 1, 2):
         tot, cnt = C.c_double(), C.c_int64()
         _lib.check(lib.tl_profile_read(op._h, kind, C.byref(tot), C.byref(cnt)))
         prof[kind] = (tot.value, cnt.value)
     iters = infos[-1]["iterations"]
     relres = infos[-1]["relres_true"]
-    # compliance of the solve against the load (f.u)
     compliance = float(torch.dot(f, u).item())
     log(f"solve: {ms:.2f} ms, {iters} PCG iterations, true relres {relres:.3e}, "
         f"compliance {compliance:.10e}")
-    log(f"fp64 operator: {prof[0][1]} launches, {prof[0][0]:.3f} ms; "
-        f"V-cycle fine operator: {prof[1][1]} launches, {prof[1][0]:.3f} ms")
-
+    vb = 4 if args.vcycle == "fp32" else 8
     flops_ku = 1152.0 * nel
-    # dominant kernel = the fine operator kind with the larger total time
-    dom = 0 if prof[0][0] >= prof[1][0] else 1
+    kinds = {
+        0: ("ku3d_kernel FP64 fused PCG product (p = z + beta p; q = K p; p.q)",
+            32 * ndof + 8 * nel, FP64_PEAK_TFLOPS),
+        1: (f"ku3d_kernel {args.vcycle.upper()} V-cycle residual (r = b - K x)",
+            (2 * vb + 8) * ndof + vb * nel, FP32_PEAK_TFLOPS if vb == 4 else FP64_PEAK_TFLOPS),
+        2: (f"ku3d_kernel {args.vcycle.upper()} V-cycle damped-Jacobi sweep (z in FP64, r.z)",
+            (2 * vb + 16) * ndof + vb * nel, FP32_PEAK_TFLOPS if vb == 4 else FP64_PEAK_TFLOPS),
+    }
+    for k, (tot, cnt) in prof.items():
+        if cnt:
+            avg = tot / cnt
+            log(f"kind {k}: {kinds[k][0]}: {cnt} launches, {tot:.3f} ms, {avg * 1e3:.1f} us/launch, "
+                f"{flops_ku / (avg * 1e-3) / 1e12:.1f} TFLOP/s, "
+                f"{kinds[k][1] / (avg * 1e-3) / 1e9:.0f} GB/s algorithmic")
+    dom = max(prof, key=lambda k: prof[k][0])
+    name, bytes_alg, peak = kinds[dom]
     avg_ms = prof[dom][0] / max(prof[dom][1], 1)
     achieved_tf = flops_ku / (avg_ms * 1e-3) / 1e12
-    if dom == 0:
-        name = "ku3d FP64 fused PCG product (p = z + beta p; q = K p; p.q)"
-        bytes_alg = 32 * ndof + 8 * nel
-        peak = FP64_PEAK_TFLOPS
-    else:
-        name = "ku3d FP32 V-cycle operator (residual / Jacobi sweep)"
-        bytes_alg = 16 * ndof + 4 * nel
-        peak = FP32_PEAK_TFLOPS
-    traffic = solve_traffic_bytes(dims, args.levels, iters, 4 if args.vcycle == "fp32" else 8)
+    traffic = solve_traffic_bytes(dims, args.levels, iters, vb)
     solve_tbps = traffic / (ms * 1e-3) / 1e12
-    log(f"dominant kernel: {name}: {avg_ms * 1e3:.1f} us/launch, {achieved_tf:.1f} TFLOP/s, "
-        f"{bytes_alg / (avg_ms * 1e-3) / 1e9:.0f} GB/s algorithmic")
+    log(f"dominant kernel: {name}: {avg_ms * 1e3:.1f} us/launch, {achieved_tf:.1f} TFLOP/s")
     log(f"solve algorithmic traffic {traffic / 1e9:.2f} GB -> {solve_tbps:.2f} TB/s")
+    hbm = None
+    tfile = ROOT / "profiles" / "traffic.json"
+    if tfile.exists():
+        tj = json.loads(tfile.read_text())
+        hbm = tj.get(f"kind{dom}_{args.vcycle}_{nelx}")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -181,7 +201,7 @@ def main():
         cpu = {"value": round(1e3 * tc * scale, 1), "unit": "ms per FEM solve (extrapolated)",
                "cores": 1, "kind": "port",
                "sample": (f"oracle numpy/scipy FP64 GMG-PCG (4 levels, CSR operators, "
-                          f"Galerkin hierarchy once per solve) on the same cantilever at 48x24x24 "
+                          f"Galerkin hierarchy once per solve) on the same cantilever at 96x48x48 "
                           f"({ndof_s} DOF): {tc:.2f} s incl. assembly, {itc} iterations; "
                           f"scaled by the DOF ratio {scale:.1f} (linear-time assumption); "
                           f"scipy sparse kernels run on 1 of {os.cpu_count()} host cores")}
@@ -208,7 +228,7 @@ def main():
                    "compliance": compliance, "solve_hbm_TBps": round(solve_tbps, 3)},
         "roofline": {"bound": "mfma", "kernel": name, "achieved": round(achieved_tf, 2),
                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4),
-                     "traffic": None, "avg_us": round(avg_ms * 1e3, 2),
+                     "traffic": hbm, "avg_us": round(avg_ms * 1e3, 2),
                      "algorithmic_bytes": bytes_alg, "algorithmic_flops": flops_ku},
         "cpu_baseline": cpu,
     }

@@ -124,8 +124,8 @@ int tl_oc_update(int64_t n, const double* x, const double* dc,
 
 /* Instrumentation (no counterpart in the reference): while enabled, every
  * launch of the fine-grid operator kernel is bracketed by HIP events on the
- * handle's stream.  kind 0: FP64 operator (PCG product, true residual),
- * kind 1: FP32/FP64 V-cycle operator (smoothing, residual).  tl_profile_read
+ * handle's stream.  kind 0: PCG products (K p, true residual), kind 1:
+ * V-cycle residual, kind 2: V-cycle Jacobi sweeps.  tl_profile_read
  * synchronises, returns the summed kernel time and launch count of a kind
  * and clears that kind. */
 int tl_profile(tl_problem* p, int32_t enable);

@@ -253,8 +253,8 @@ __device__ inline void stage_plane(const KuArgs& a, T* lds, int pz, int64_t j0, 
   }
 }
 
-template <int NDPN, typename T, int SLOT>
-__device__ inline void read_nbrs(const KuArgs& a, const T* lds, T (&U)[3][9][NDPN]) {
+template <int NDPN, typename T, int SLOT, int NS>
+__device__ inline void read_nbrs(const KuArgs& a, const T* lds, T (&U)[NS][9][NDPN]) {
   const int c0 = threadIdx.x + a.ny + 1;
 #pragma unroll
   for (int dx = -1; dx <= 1; ++dx)
@@ -276,30 +276,208 @@ __device__ inline void load_dens(const KuArgs& a, int ix, int iy, int ez, T (&S)
       S[(ox + 1) * 2 + (oy + 1)] = sp[(rowz + ix + ox + 1) * (a.nely + 2) + iy + oy + 1];
 }
 
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST,
-          int SA, int SB, int SC>
-__device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t j0,
-                              int64_t j, int ix, int iy, bool active, T (&U)[3][9][NDPN],
-                              T (&S)[3][4], double& dot) {
-  // plane iz+1 -> slot SC (LDS buffer by parity), element layer iz -> S[SB]
-  T* buf = ((iz + 1) & 1) ? lds1 : lds0;
-  stage_plane<NDPN, T>(a, buf, iz + 1, j0, FUSEP);
-  load_dens<T>(a, ix, iy, iz, S[SB]);
-  __syncthreads();
-  read_nbrs<NDPN, T, SC>(a, buf, U);
-  T acc[NDPN];
-  node_sum<NDPN, T, SA, SB, SC, true, FAST>(a, U, S, SA, SB, acc);
-  if (active) {
-    const int64_t node = int64_t(iz) * a.plane + j;
-    T uc[NDPN];
+// Register prefetch of one staged plane: slot k holds LDS entry
+// threadIdx.x + k*BS.  PF covers planes with (BS + 2*ny + 2)*NDPN <= PF*BS
+// (ny <= 169 for NDPN = 3); entries beyond are staged synchronously.
+template <int NDPN>
+struct PfCount {
+  static constexpr int value = NDPN == 3 ? 7 : (NDPN == 2 ? 5 : 3);
+};
+
+template <int NDPN, typename T, bool FUSEP>
+struct PlanePrefetch {
+  static constexpr int PF = PfCount<NDPN>::value;
+  T a[PF];
+  T b[FUSEP ? PF : 1];
+};
+
+template <int NDPN, typename T, bool FUSEP>
+__device__ inline void fetch_plane(const KuArgs& a, PlanePrefetch<NDPN, T, FUSEP>& pf, int pz,
+                                   int64_t j0) {
+  constexpr int PF = PfCount<NDPN>::value;
+  const int L = BS + 2 * a.ny + 2;
+  const int64_t base = j0 - a.ny - 1;
+  const bool zin = pz >= 0 && pz < a.nz;
 #pragma unroll
-    for (int d = 0; d < NDPN; ++d) uc[d] = U[SB][4][d];
+  for (int k = 0; k < PF; ++k) {
+    const int t = threadIdx.x + k * BS;
+    const int64_t jj = base + t / NDPN;
+    T va = T(0), vb = T(0);
+    if (zin && t < L * NDPN && jj >= 0 && jj < a.plane) {
+      const int64_t g = (int64_t(pz) * a.plane + jj) * NDPN + (t % NDPN);
+      va = ldval<T>(a.u, g);
+      if constexpr (FUSEP) vb = ldval<T>(a.pin, g);
+    }
+    pf.a[k] = va;
+    if constexpr (FUSEP) pf.b[k] = vb;
+  }
+}
+
+template <int NDPN, typename T, bool FUSEP>
+__device__ inline void store_plane(const KuArgs& a, const PlanePrefetch<NDPN, T, FUSEP>& pf,
+                                   T* lds, int pz, int64_t j0) {
+  constexpr int PF = PfCount<NDPN>::value;
+  const int L = BS + 2 * a.ny + 2;
+  T beta = T(0);
+  if constexpr (FUSEP) beta = T(*a.beta);
+#pragma unroll
+  for (int k = 0; k < PF; ++k) {
+    const int t = threadIdx.x + k * BS;
+    if (t < L * NDPN) lds[t] = FUSEP ? pf.a[k] + beta * pf.b[k] : pf.a[k];
+  }
+  // entries past the prefetch slots (very wide planes): staged directly
+  const int64_t base = j0 - a.ny - 1;
+  const bool zin = pz >= 0 && pz < a.nz;
+  for (int t = threadIdx.x + PF * BS; t < L * NDPN; t += BS) {
+    const int64_t jj = base + t / NDPN;
+    T v = T(0);
+    if (zin && jj >= 0 && jj < a.plane)
+      v = stage_val<T>(a, (int64_t(pz) * a.plane + jj) * NDPN + (t % NDPN), FUSEP);
+    lds[t] = v;
+  }
+}
+
+// Epilogue inputs of one node, fetched at the start of its step.
+template <int NDPN, typename T, int MODE, typename TB>
+struct EpiPrefetch {
+  TB b[NDPN];
+  T dinv[NDPN];
+  uint8_t mask;
+};
+
+template <int NDPN, typename T, int MODE, typename TB>
+__device__ inline void fetch_epi(const KuArgs& a, int64_t node, EpiPrefetch<NDPN, T, MODE, TB>& e) {
+  e.mask = a.mask[node];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) {
+    if constexpr (MODE == KU_RESID || MODE == KU_JACOBI || MODE == KU_JACOBI_DOT)
+      e.b[d] = ldval<TB>(a.b, node * NDPN + d);
+    if constexpr (MODE == KU_JACOBI || MODE == KU_JACOBI_DOT)
+      e.dinv[d] = ldval<T>(a.dinv, node * NDPN + d);
+  }
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO>
+__device__ inline void ku_epilogue_pf(const KuArgs& a, int64_t node,
+                                      const EpiPrefetch<NDPN, T, MODE, TB>& e,
+                                      const T (&uc)[NDPN], const T (&acc)[NDPN], double& dot) {
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) {
+    const int64_t g = node * NDPN + d;
+    const bool fixed = (e.mask >> d) & 1;
+    if constexpr (MODE == KU_STORE || MODE == KU_STORE_DOT) {
+      const T v = fixed ? T(0) : acc[d];
+      static_cast<TO*>(a.y)[g] = TO(v);
+      if constexpr (MODE == KU_STORE_DOT) dot += double(uc[d]) * double(v);
+    } else if constexpr (MODE == KU_RESID) {
+      const T v = fixed ? T(0) : T(e.b[d]) - acc[d];
+      static_cast<TO*>(a.y)[g] = TO(v);
+    } else {
+      const T bb = T(e.b[d]);
+      const T v = fixed ? T(0) : uc[d] + T(a.omega) * e.dinv[d] * (bb - acc[d]);
+      static_cast<TO*>(a.y)[g] = TO(v);
+      if constexpr (MODE == KU_JACOBI_DOT) dot += double(bb) * double(v);
+    }
+  }
+}
+
+// Contributions of the 4 elements of one element layer (between node planes
+// iz and iz+1, neighbour values in U slots SA and SB) to the thread's node on
+// plane iz (acc_top: the layer lies above it) and on plane iz+1 (acc_bot: the
+// layer lies below it).  DO_TOP / DO_BOT select the halves (chunk ends).
+template <int NDPN, typename T, int SA, int SB, bool FAST, bool DO_TOP, bool DO_BOT>
+__device__ inline void layer_sum(const KuArgs& a, const T (&U)[2][9][NDPN], const T (&S)[4],
+                                 T (&acc_top)[NDPN], T (&acc_bot)[NDPN]) {
+  constexpr int ND = 8 * NDPN;
+  int opq;
+  asm volatile("s_mov_b32 %0, 0" : "=s"(opq));
+  const __attribute__((address_space(4))) T* KE =
+      (const __attribute__((address_space(4))) T*)(static_cast<const T*>(a.KE)) + opq;
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) acc_top[d] = acc_bot[d] = T(0);
+#pragma unroll
+  for (int ox = -1; ox <= 0; ++ox) {
+#pragma unroll
+    for (int oy = -1; oy <= 0; ++oy) {
+      const T s = S[(ox + 1) * 2 + (oy + 1)];
+      const int la_top = loc_of(-ox, -oy, 0), la_bot = loc_of(-ox, -oy, 1);
+      T tt[NDPN], tb[NDPN];
+#pragma unroll
+      for (int d = 0; d < NDPN; ++d) tt[d] = tb[d] = T(0);
+#pragma unroll
+      for (int lb = 0; lb < 8; ++lb) {
+        const int slot = loc_dz(lb) == 0 ? SA : SB;
+        const int nb = (ox + loc_dx(lb) + 1) * 3 + (oy + loc_dy(lb) + 1);
+#pragma unroll
+        for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+          for (int c = 0; c < NDPN; ++c) {
+            const int et = (la_top * NDPN + d) * ND + lb * NDPN + c;
+            const int eb = (la_bot * NDPN + d) * ND + lb * NDPN + c;
+            const T uv = U[slot][nb][c];
+            if constexpr (FAST) {
+              const int pt = ke_pat<true, NDPN>(et), pb = ke_pat<true, NDPN>(eb);
+              const T kt = kv_at<T>(a, (pt > 0 ? pt : -pt) - 1);
+              const T kb = kv_at<T>(a, (pb > 0 ? pb : -pb) - 1);
+              if constexpr (DO_TOP) tt[d] = fma_t(pt > 0 ? kt : -kt, uv, tt[d]);
+              if constexpr (DO_BOT) tb[d] = fma_t(pb > 0 ? kb : -kb, uv, tb[d]);
+            } else {
+              if constexpr (DO_TOP) tt[d] = fma_t(T(KE[et]), uv, tt[d]);
+              if constexpr (DO_BOT) tb[d] = fma_t(T(KE[eb]), uv, tb[d]);
+            }
+          }
+      }
+#pragma unroll
+      for (int d = 0; d < NDPN; ++d) {
+        if constexpr (DO_TOP) acc_top[d] = fma_t(s, tt[d], acc_top[d]);
+        if constexpr (DO_BOT) acc_bot[d] = fma_t(s, tb[d], acc_bot[d]);
+      }
+    }
+  }
+}
+
+// One step over element layer iz (node planes iz, iz+1).  On entry: U[SA]
+// holds plane iz, the LDS buffer of plane iz+1 is written (barrier pending),
+// S holds layer iz's densities and `carry` the layer-(iz-1) part of node iz.
+// The step prefetches plane iz+2 and layer iz+1 into registers, completes
+// and writes node iz, carries node iz+1's partial sum, then stores plane
+// iz+2 into the other LDS buffer.
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST,
+          int SA, int SB>
+__device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t j0,
+                              int64_t j, int ix, int iy, bool active, T (&U)[2][9][NDPN],
+                              T (&S)[4], T (&carry)[NDPN], double& dot) {
+  T* cur = ((iz + 1) & 1) ? lds1 : lds0;  // holds plane iz+1
+  T* nxt = ((iz + 2) & 1) ? lds1 : lds0;  // receives plane iz+2
+  __syncthreads();
+  read_nbrs<NDPN, T, SB>(a, cur, U);
+  PlanePrefetch<NDPN, T, FUSEP> pf;
+  fetch_plane<NDPN, T, FUSEP>(a, pf, iz + 2, j0);
+  const int64_t node = int64_t(iz) * a.plane + (active ? j : 0);
+  EpiPrefetch<NDPN, T, MODE, TB> ep;
+  fetch_epi<NDPN, T, MODE, TB>(a, node, ep);
+  T Sn[4];
+  load_dens<T>(a, ix, iy, iz + 1, Sn);
+  T acc_top[NDPN], acc_bot[NDPN];
+  layer_sum<NDPN, T, SA, SB, FAST, true, true>(a, U, S, acc_top, acc_bot);
+  if (active) {
+    T uc[NDPN], tot[NDPN];
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d) {
+      uc[d] = U[SA][4][d];
+      tot[d] = carry[d] + acc_top[d];
+    }
     if constexpr (FUSEP) {
 #pragma unroll
       for (int d = 0; d < NDPN; ++d) static_cast<T*>(a.pout)[node * NDPN + d] = uc[d];
     }
-    ku_epilogue<NDPN, T, MODE, TB, TO>(a, node, uc, acc, dot);
+    ku_epilogue_pf<NDPN, T, MODE, TB, TO>(a, node, ep, uc, tot, dot);
   }
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) carry[d] = acc_bot[d];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) S[k] = Sn[k];
+  store_plane<NDPN, T, FUSEP>(a, pf, nxt, iz + 2, j0);
 }
 
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
@@ -309,34 +487,48 @@ __global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
   const int L = BS + 2 * a.ny + 2;
   T* lds0 = reinterpret_cast<T*>(lds_raw);
   T* lds1 = lds0 + L * NDPN;
-  const int64_t j0 = int64_t(blockIdx.x) * BS;
+  // XCD-aware tile order: workgroups are dispatched round-robin over the 8
+  // XCDs, each with its own L2; remap so that XCD x runs a contiguous range
+  // of tiles (plane chunks fastest) and the halos shared by neighbouring
+  // tiles are re-read from its own L2 instead of HBM.
+  const unsigned nb = gridDim.x * gridDim.y;
+  const unsigned hw = blockIdx.y * gridDim.x + blockIdx.x;
+  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
+  const unsigned bx = logical % gridDim.x, by = logical / gridDim.x;
+  const int64_t j0 = int64_t(bx) * BS;
   const int64_t j = j0 + threadIdx.x;
   const bool inplane = j < a.plane;
   const int64_t jc = inplane ? j : a.plane - 1;
   const int ix = int(jc / a.ny), iy = int(jc % a.ny);
-  const int z0 = blockIdx.y * a.zchunk;
+  const int z0 = by * a.zchunk;
   const int z1 = min(a.nz, z0 + a.zchunk);
-  T U[3][9][NDPN];
-  T S[3][4];
+  T U[2][9][NDPN];
+  T S[4];
+  T carry[NDPN];
   double dot = 0.0;
-  // prologue: planes z0-1 (slot 0) and z0 (slot 1), element layer z0-1 (S[0])
-  stage_plane<NDPN, T>(a, lds0, z0 - 1, j0, FUSEP);
-  stage_plane<NDPN, T>(a, lds1, z0, j0, FUSEP);
-  load_dens<T>(a, ix, iy, z0 - 1, S[0]);
+  // prologue: planes z0-1 -> U[0], z0 -> U[1]; layer z0-1 gives node z0's
+  // "below" part; plane z0+1 staged for the first step; layer z0 densities
+  T* b_m = ((z0 + 1) & 1) ? lds1 : lds0;  // parity of z0-1 == parity of z0+1
+  T* b_c = (z0 & 1) ? lds1 : lds0;
+  stage_plane<NDPN, T>(a, b_m, z0 - 1, j0, FUSEP);
+  stage_plane<NDPN, T>(a, b_c, z0, j0, FUSEP);
+  load_dens<T>(a, ix, iy, z0 - 1, S);
   __syncthreads();
-  read_nbrs<NDPN, T, 0>(a, lds0, U);
-  read_nbrs<NDPN, T, 1>(a, lds1, U);
+  read_nbrs<NDPN, T, 0>(a, b_m, U);
+  read_nbrs<NDPN, T, 1>(a, b_c, U);
+  {
+    T unused[NDPN];
+    layer_sum<NDPN, T, 0, 1, FAST, false, true>(a, U, S, unused, carry);
+  }
   __syncthreads();
-  // planes of node iz use LDS buffer (iz & 1); rotating slots (0,1,2)
-  for (int iz = z0; iz < z1; iz += 3) {
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 0, 1, 2>(a, lds0, lds1, iz, j0, j, ix, iy, inplane, U, S, dot);
-    __syncthreads();
+  stage_plane<NDPN, T>(a, b_m, z0 + 1, j0, FUSEP);
+  load_dens<T>(a, ix, iy, z0, S);
+  // alternating U slots: plane iz in slot SA, plane iz+1 read into SB
+  for (int iz = z0; iz < z1; iz += 2) {
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 1, 0>(a, lds0, lds1, iz, j0, j, ix, iy, inplane, U, S, carry, dot);
     if (iz + 1 >= z1) break;
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 1, 2, 0>(a, lds0, lds1, iz + 1, j0, j, ix, iy, inplane, U, S, dot);
-    __syncthreads();
-    if (iz + 2 >= z1) break;
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 2, 0, 1>(a, lds0, lds1, iz + 2, j0, j, ix, iy, inplane, U, S, dot);
-    __syncthreads();
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 0, 1>(a, lds0, lds1, iz + 1, j0, j, ix, iy, inplane, U, S, carry, dot);
   }
   if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
     __shared__ double sh[BS / 64 + 2];
@@ -403,8 +595,10 @@ static int launch_ku_impl(const tl_problem* P, KuArgs a) {
   const unsigned nbx = unsigned((int64_t(a.plane) + BS - 1) / BS);
   if (g.dim == 3) {
     // z chunk: enough workgroups to fill 256 CUs, each chunk >= 4 planes
+    // z chunk: ~1024 workgroups (4 per CU), chunks of >= 6 planes so the
+    // z halo (2 extra planes per chunk) stays small
     int zc = g.nz;
-    while (zc > 4 && int64_t(nbx) * ((g.nz + zc - 1) / zc) < 2048) zc = (zc + 1) / 2;
+    while (zc > 6 && int64_t(nbx) * ((g.nz + zc - 1) / zc) < 1024) zc = (zc + 1) / 2;
     a.zchunk = zc;
     dim3 grid(nbx, unsigned((g.nz + zc - 1) / zc));
     if (int64_t(grid.x) * grid.y > kMaxRedBlocks && (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)) {
@@ -501,8 +695,9 @@ int fine_apply_ex(tl_problem* P, const void* u, void* y, int prec, int mode, con
       ev = P->prof_free.back();
       P->prof_free.pop_back();
     }
-    // kind 0: fp64 products of the PCG loop; kind 1: operator inside the V-cycle
-    ev.kind = (mode == KU_STORE || mode == KU_STORE_DOT) ? 0 : 1;
+    // kind 0: products of the PCG loop (K p, true residual); kind 1: V-cycle
+    // residual; kind 2: V-cycle Jacobi sweeps
+    ev.kind = (mode == KU_STORE || mode == KU_STORE_DOT) ? 0 : (mode == KU_RESID ? 1 : 2);
     TL_HIP(hipEventRecord(ev.start, P->stream));
   }
   const int rc = fine_apply_dispatch(P, a, prec, mode, fusep, b64, out64);
@@ -985,10 +1180,18 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
   double* pp[2] = {p, P->p2.as<double>()};
   TL_HIP(hipMemsetAsync(pp[0], 0, sizeof(double) * n, s));
   TL_HIP(hipMemsetAsync(pp[1], 0, sizeof(double) * n, s));
-  const int chunk = std::max(1, o->check_every);
+  // Iterations run in chunks between host checks of the device state; after
+  // the first chunk the chunk length follows the observed residual decay so
+  // that few iterations are launched past convergence (they exit at once but
+  // still cost a launch each).
+  int chunk = std::max(1, o->check_every);
   PcgState hs{};
+  TL_HIP(hipMemcpyAsync(&hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  TL_HIP(hipStreamSynchronize(s));
+  double rr_prev = hs.rr;
+  int it_prev = 0;
   int64_t kk = 0;
-  for (int launched = 0;;) {
+  for (int launched = 0; !hs.done;) {
     for (int k = 0; k < chunk; ++k) {
       // z = M r ; rho = r.z ; beta
       if (o->precond == TL_PRECOND_GMG) {
@@ -1017,6 +1220,15 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
     TL_HIP(hipStreamSynchronize(s));
     if (hs.done) break;
     if (launched > o->maxiter + chunk) break;  // defensive: done must have been set
+    if (hs.it > it_prev && hs.rr > 0.0 && hs.rr < rr_prev && hs.tol2 > 0.0) {
+      const double rate = std::log(hs.rr / rr_prev) / double(hs.it - it_prev);
+      const double rem = std::log(hs.tol2 / hs.rr) / rate;
+      chunk = int(std::min(64.0, std::max(1.0, std::floor(0.9 * rem))));
+    } else {
+      chunk = std::max(1, o->check_every);
+    }
+    rr_prev = hs.rr;
+    it_prev = hs.it;
   }
   // true residual in fp64
   TL_TRY(fine_apply_ex(P, u, q, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0, nullptr,
