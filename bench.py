@@ -8,7 +8,7 @@ load vector) are resident in HBM before the timed region.  Prints one JSON
 line; extra diagnostics go to stderr.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--levels 5]
-                       [--vcycle fp32|fp64] [--no-cpu-baseline]
+                       [--vcycle fp32|fp64|bf16|fp16] [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -92,7 +92,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--levels", type=int, default=5)
-    ap.add_argument("--vcycle", default="fp32", choices=["fp32", "fp64"])
+    ap.add_argument("--vcycle", default="fp32", choices=["fp32", "fp64", "bf16", "fp16"])
     ap.add_argument("--nelx", type=int, default=192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -164,15 +164,21 @@ def main():
     compliance = float(torch.dot(f, u).item())
     log(f"solve: {ms:.2f} ms, {iters} PCG iterations, true relres {relres:.3e}, "
         f"compliance {compliance:.10e}")
-    vb = 4 if args.vcycle == "fp32" else 8
+    vb = 8 if args.vcycle == "fp64" else 4  # V-cycle vector bytes (bf16/fp16: fp32 vectors)
+    mfma = args.vcycle in ("bf16", "fp16")
     flops_ku = 1152.0 * nel
+    vpeak = {"fp32": FP32_PEAK_TFLOPS, "fp64": FP64_PEAK_TFLOPS}.get(args.vcycle, HBM_PEAK_GBS)
+    vkern = (f"kum3d_kernel {args.vcycle.upper()}-input MFMA" if mfma
+             else f"ku3d_kernel {args.vcycle.upper()}")
+    # (name, algorithmic bytes, peak, bound): the MFMA operator's flops are
+    # far below the matrix cores' peak, so it is priced against HBM
     kinds = {
         0: ("ku3d_kernel FP64 fused PCG product (p = z + beta p; q = K p; p.q)",
-            32 * ndof + 8 * nel, FP64_PEAK_TFLOPS),
-        1: (f"ku3d_kernel {args.vcycle.upper()} V-cycle residual (r = b - K x)",
-            (2 * vb + 8) * ndof + vb * nel, FP32_PEAK_TFLOPS if vb == 4 else FP64_PEAK_TFLOPS),
-        2: (f"ku3d_kernel {args.vcycle.upper()} V-cycle damped-Jacobi sweep (z in FP64, r.z)",
-            (2 * vb + 16) * ndof + vb * nel, FP32_PEAK_TFLOPS if vb == 4 else FP64_PEAK_TFLOPS),
+            32 * ndof + 8 * nel, FP64_PEAK_TFLOPS, "mfma"),
+        1: (f"{vkern} V-cycle residual (r = b - K x)",
+            (2 * vb + 8) * ndof + vb * nel + (ndof // 3), vpeak, "hbm" if mfma else "mfma"),
+        2: (f"{vkern} V-cycle damped-Jacobi sweep (z in FP64, r.z)",
+            (2 * vb + 16) * ndof + vb * nel + (ndof // 3), vpeak, "hbm" if mfma else "mfma"),
     }
     for k, (tot, cnt) in prof.items():
         if cnt:
@@ -181,9 +187,10 @@ def main():
                 f"{flops_ku / (avg * 1e-3) / 1e12:.1f} TFLOP/s, "
                 f"{kinds[k][1] / (avg * 1e-3) / 1e9:.0f} GB/s algorithmic")
     dom = max(prof, key=lambda k: prof[k][0])
-    name, bytes_alg, peak = kinds[dom]
+    name, bytes_alg, peak, bound = kinds[dom]
     avg_ms = prof[dom][0] / max(prof[dom][1], 1)
     achieved_tf = flops_ku / (avg_ms * 1e-3) / 1e12
+    achieved_gbs = bytes_alg / (avg_ms * 1e-3) / 1e9
     traffic = solve_traffic_bytes(dims, args.levels, iters, vb)
     solve_tbps = traffic / (ms * 1e-3) / 1e12
     log(f"dominant kernel: {name}: {avg_ms * 1e3:.1f} us/launch, {achieved_tf:.1f} TFLOP/s")
@@ -218,7 +225,10 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64 outer PCG + " + ("f32" if args.vcycle == "fp32" else "f64") + " V-cycle",
+        "dtype": "f64 outer PCG + " + {"fp32": "f32", "fp64": "f64",
+                                        "bf16": "bf16-input/f32-accumulate MFMA",
+                                        "fp16": "f16-input/f32-accumulate MFMA"}[args.vcycle]
+                 + " V-cycle",
         "data": "synthetic (seeded Gaussian random density field, sigma 4, mean 0.3, seed 0)",
         "config": {"workload": f"cantilever3d H8 {nelx}x{nely}x{nelz} ({ndof} DOF), x=0 face "
                                f"clamped, unit line load, GMG-PCG to relres 1e-8",
@@ -226,10 +236,13 @@ def main():
                    "omega": 0.6, "smoothing": "V(1,1) damped Jacobi",
                    "pcg_iterations": iters, "relres_true": relres,
                    "compliance": compliance, "solve_hbm_TBps": round(solve_tbps, 3)},
-        "roofline": {"bound": "mfma", "kernel": name, "achieved": round(achieved_tf, 2),
-                     "peak": peak, "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4),
-                     "traffic": hbm, "avg_us": round(avg_ms * 1e3, 2),
-                     "algorithmic_bytes": bytes_alg, "algorithmic_flops": flops_ku},
+        "roofline": ({"bound": "mfma", "kernel": name, "achieved": round(achieved_tf, 2),
+                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4)}
+                     if bound == "mfma" else
+                     {"bound": "hbm", "kernel": name, "achieved": round(achieved_gbs, 1),
+                      "peak": peak, "unit": "GB/s", "frac": round(achieved_gbs / peak, 4)})
+        | {"traffic": hbm, "avg_us": round(avg_ms * 1e3, 2),
+           "algorithmic_bytes": bytes_alg, "algorithmic_flops": flops_ku},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

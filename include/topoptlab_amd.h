@@ -58,7 +58,10 @@ int tl_set_density(tl_problem* p, const double* xPhys);
 
 /* y = K_ff u (fixed rows/cols removed), matrix-free.  Replaces the CSR
  * product of topology_optimization.py:429-451 + linear_solvers.py:114,130.
- * prec: TL_PREC_FP64 or TL_PREC_FP32 (inputs/outputs stay float64). */
+ * prec: TL_PREC_FP64 or TL_PREC_FP32; TL_PREC_BF16 / TL_PREC_FP16 run the
+ * V-cycle's matrix-core operator (3D H8 elasticity only: u rounded to fp32,
+ * element vectors and KE to bf16/fp16 after max-normalisation, fp32
+ * accumulation).  Inputs/outputs stay float64. */
 int tl_apply_K(tl_problem* p, const double* u, double* y, int32_t prec);
 
 /* Diagonal of K_ff (fixed DOFs: 0). */
@@ -68,7 +71,9 @@ typedef struct {
   int32_t nlevels;       /* grid levels incl. the fine one (>= 2) */
   double omega;          /* damped-Jacobi weight */
   int32_t nu1, nu2;      /* pre-/post-smoothing sweeps */
-  int32_t precision;     /* V-cycle arithmetic: TL_PREC_FP64 or TL_PREC_FP32 */
+  int32_t precision;     /* V-cycle arithmetic: TL_PREC_FP64, TL_PREC_FP32, or
+                            TL_PREC_BF16 / TL_PREC_FP16 (3D H8 elasticity: fine-
+                            level operator on MFMA, fp32 vectors and coarse levels) */
 } tl_mg_opts;
 
 /* Galerkin GMG hierarchy for the current density (multigrid.py:89-223,

@@ -710,6 +710,15 @@ int fine_apply_ex(tl_problem* P, const void* u, void* y, int prec, int mode, con
 
 static int fine_apply_dispatch(tl_problem* P, KuArgs& a, int prec, int mode, bool fusep, bool b64,
                                bool out64) {
+  if (prec == TL_PREC_BF16 || prec == TL_PREC_FP16) {
+    // V-cycle smoothing and residual on the matrix cores
+    TL_ARG(b64 && !fusep && (mode == KU_RESID || mode == KU_JACOBI || mode == KU_JACOBI_DOT),
+           "internal: the MFMA operator serves the V-cycle modes with an fp64 right-hand side");
+    unsigned nb = 0;
+    TL_TRY(fine_apply_mfma(P, a.u, a.y, prec, mode, a.b, a.dinv, a.omega, a.partials, a.done, &nb));
+    if (mode == KU_JACOBI_DOT) return launch_fold(P, nb, a.fin, a.dot_out, a.done);
+    return TL_OK;
+  }
   if (prec == TL_PREC_FP64) {
     a.sp = P->op.sp64.ptr; a.KE = P->op.KE64.ptr;
     switch (P->ndpn) {
@@ -906,6 +915,12 @@ __global__ void cast_kernel_f2d(const float* __restrict__ a, double* __restrict_
     b[i] = double(a[i]);
 }
 
+__global__ void neg_cast_kernel_f2d(const float* __restrict__ a, double* __restrict__ b, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    b[i] = -double(a[i]);
+}
+
 __global__ void init_state_kernel(PcgState* st, double rtol, double atol, int maxiter) {
   // tolerance as linear_solvers.py:107-109 (scipy _get_atol_rtol): max(atol, rtol*||b||)
   const double tol = fmax(atol, rtol * sqrt(st->bb));
@@ -1068,7 +1083,11 @@ int tl_set_density(tl_problem* P, const double* xPhys) {
 int tl_apply_K(tl_problem* P, const double* u, double* y, int32_t prec) {
   TL_ARG(P && u && y, "null argument");
   TL_ARG(P->op.have_density, "tl_set_density must be called first");
-  TL_ARG(prec == TL_PREC_FP64 || prec == TL_PREC_FP32, "prec must be FP64 or FP32");
+  TL_ARG(prec == TL_PREC_FP64 || prec == TL_PREC_FP32 || prec == TL_PREC_BF16 ||
+             prec == TL_PREC_FP16,
+         "prec must be FP64, FP32, BF16 or FP16");
+  TL_ARG(prec == TL_PREC_FP64 || prec == TL_PREC_FP32 || (P->g.dim == 3 && P->ndpn == 3),
+         "BF16/FP16 (MFMA) products are implemented for 3D H8 elasticity only");
   TL_TRY(P->t.alloc(sizeof(double) * P->ndof));
   const unsigned nb = grid_for(P->ndof);
   hipLaunchKernelGGL(masked_copy_kernel, dim3(nb), dim3(BS), 0, P->stream, u, P->t.as<double>(),
@@ -1084,9 +1103,19 @@ int tl_apply_K(tl_problem* P, const double* u, double* y, int32_t prec) {
   hipLaunchKernelGGL(cast_kernel_d2f, dim3(nb), dim3(BS), 0, P->stream, P->t.as<double>(), u32,
                      P->ndof);
   TL_CHECK_LAUNCH();
-  TL_TRY(fine_apply_ex(P, u32, y32, TL_PREC_FP32, KU_STORE, nullptr, false, nullptr, 0.0,
-                       nullptr, nullptr, nullptr, nullptr, FIN_STORE, nullptr, false));
-  hipLaunchKernelGGL(cast_kernel_f2d, dim3(nb), dim3(BS), 0, P->stream, y32, y, P->ndof);
+  if (prec == TL_PREC_FP32) {
+    TL_TRY(fine_apply_ex(P, u32, y32, TL_PREC_FP32, KU_STORE, nullptr, false, nullptr, 0.0,
+                         nullptr, nullptr, nullptr, nullptr, FIN_STORE, nullptr, false));
+    hipLaunchKernelGGL(cast_kernel_f2d, dim3(nb), dim3(BS), 0, P->stream, y32, y, P->ndof);
+    TL_CHECK_LAUNCH();
+    return TL_OK;
+  }
+  // bf16/fp16: the MFMA operator's residual mode with b = 0 gives -K u
+  TL_TRY(P->q.alloc(sizeof(double) * P->ndof));
+  TL_HIP(hipMemsetAsync(P->q.ptr, 0, sizeof(double) * P->ndof, P->stream));
+  TL_TRY(fine_apply_ex(P, u32, y32, prec, KU_RESID, P->q.ptr, true, nullptr, 0.0, nullptr,
+                       nullptr, nullptr, nullptr, 0, nullptr, false));
+  hipLaunchKernelGGL(neg_cast_kernel_f2d, dim3(nb), dim3(BS), 0, P->stream, y32, y, P->ndof);
   TL_CHECK_LAUNCH();
   return TL_OK;
 }

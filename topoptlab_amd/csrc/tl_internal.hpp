@@ -230,6 +230,10 @@ int fine_apply_ex(tl_problem* P, const void* u, void* y, int prec, int mode, con
                   bool out64);
 int fine_diag(tl_problem* P);
 int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done);
+// V-cycle fine operator on MFMA (3D H8 elasticity, prec TL_PREC_BF16/FP16)
+int fine_apply_mfma(tl_problem* P, const void* u, void* y, int prec, int mode, const void* b,
+                    const void* dinv, double omega, double* partials, const int* done,
+                    unsigned* nblocks_out);
 }  // namespace tl
 
 // modes of the fine matrix-free operator epilogue

@@ -605,7 +605,10 @@ int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done) {
   TL_NONNULL(L.vx.ptr); TL_NONNULL(L.vt.ptr); TL_NONNULL(L.vr.ptr); TL_NONNULL(C.vb.ptr);
   TL_NONNULL(C.vx.ptr); TL_NONNULL(C.mask.ptr); TL_NONNULL(P->op.mask.ptr);
   TL_NONNULL(P->op.dinv32.ptr); TL_NONNULL(P->op.dinv64.ptr); TL_NONNULL(r); TL_NONNULL(z);
-  const int prec = std::is_same<T, double>::value ? TL_PREC_FP64 : TL_PREC_FP32;
+  // fine-level operator: fp64, fp32, or MFMA with bf16/fp16 inputs (fp32 vectors)
+  const int prec = std::is_same<T, double>::value
+                       ? TL_PREC_FP64
+                       : ((M.prec == TL_PREC_BF16 || M.prec == TL_PREC_FP16) ? M.prec : TL_PREC_FP32);
   const void* dinv = std::is_same<T, double>::value ? P->op.dinv64.ptr : P->op.dinv32.ptr;
   T* x = vp<T>(L.vx);
   T* t = vp<T>(L.vt);
@@ -664,8 +667,12 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
   TL_ARG(o->nlevels >= 2 && o->nlevels <= 12, "nlevels must be in [2, 12]");
   TL_ARG(o->nu1 >= 1 && o->nu2 >= 1, "nu1 and nu2 must be >= 1");
   TL_ARG(o->omega > 0.0, "omega must be positive");
-  TL_ARG(o->precision == TL_PREC_FP64 || o->precision == TL_PREC_FP32,
-         "V-cycle precision must be FP64 or FP32");
+  TL_ARG(o->precision == TL_PREC_FP64 || o->precision == TL_PREC_FP32 ||
+             o->precision == TL_PREC_BF16 || o->precision == TL_PREC_FP16,
+         "V-cycle precision must be FP64, FP32, BF16 or FP16");
+  TL_ARG(o->precision == TL_PREC_FP64 || o->precision == TL_PREC_FP32 ||
+             (P->g.dim == 3 && P->ndpn == 3),
+         "BF16/FP16 (MFMA) V-cycles are implemented for 3D H8 elasticity only");
   const int L = o->nlevels;
   const int f = 1 << (L - 1);
   const Grid& g = P->g;

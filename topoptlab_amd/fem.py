@@ -21,7 +21,8 @@ import torch
 from . import _lib
 from ._lib import check, lib
 
-PRECISIONS = {"fp64": _lib.TL_PREC_FP64, "fp32": _lib.TL_PREC_FP32}
+PRECISIONS = {"fp64": _lib.TL_PREC_FP64, "fp32": _lib.TL_PREC_FP32,
+              "bf16": _lib.TL_PREC_BF16, "fp16": _lib.TL_PREC_FP16}
 PRECONDITIONERS = {None: _lib.TL_PRECOND_NONE, "none": _lib.TL_PRECOND_NONE,
                    "jacobi": _lib.TL_PRECOND_JACOBI, "gmg": _lib.TL_PRECOND_GMG}
 
