@@ -1,48 +1,63 @@
 // V-cycle fine-grid operator of 3D H8 elasticity on MFMA (BF16 or FP16
-// inputs, FP32 accumulation): the per-element contraction
-//   Y_e = KE (24x24) * u_e (24)
-// of topoptlab's assembly (topology_optimization.py:419-431, fem.py:141-178)
-// batched 16 elements at a time as the columns of
-// v_mfma_f32_16x16x32_{bf16,f16} (KE padded to 32x32: two 16-row tiles, one
-// 32-deep K step), then scaled by s_e and summed into the nodes.
+// inputs, FP32 accumulation): the element contraction Y_e = s_e KE u_e of
+// topoptlab's assembly (topology_optimization.py:419-431, fem.py:141-178),
+// summed into the nodes without assembling anything.
 //
-// Precision: each element vector is normalised by its max |u| and KE by its
-// max |entry| before the conversion, so the 1e9 SIMP contrast, the spread of
-// the iterate's magnitude and the material's units stay out of the 8-/11-bit
-// mantissas and the FP16 range; the FP32 factor s_e*max|u_e|*max|KE| is
-// applied to the FP32 MFMA result.
+// Formulation ("column walk").  A wavefront owns a chunk of 16 consecutive
+// nodes along y (MFMA column m <-> node gy = y0-1+m; columns 0 and 15 are
+// the chunk's halo, so 14 nodes are produced per chunk), a tile of TX node
+// lines in x and a range of element layers in z.  For one element column
+// (ex, ez) two v_mfma_f32_16x16x32 products
+//     D0 = A0 * B0   (element (ex, gy, ez))
+//     D1 = A1 * B1   (element (ex, gy-1, ez))
+// give, for every column m, the contributions of the two elements that
+// share node gy to the 4 node lines (ex+dx, ez+dz) of that node:
+//   * rows of A/D: 4 node lines (dx, dz) x 3 components (+1 zero row),
+//   * K of A/B (32): 4 node lines (dx', dz') x [2 nodes along y, 3
+//     components] (6 entries, 2 zero); each lane's 8 B0 entries are the
+//     values of one node line at nodes gy, gy+1, and B1 (nodes gy-1, gy) is
+//     B0 of the neighbouring column, moved over by one DPP row shift;
+//   * A0 / A1 are the KE rows of local node (dx, dy=0 / 1, dz).
+// (BF16: s_e inside B, one accumulator; FP16: y = g0*D0 + g1*D1.)
+// Each lane loads one node (12 bytes) and one element scale per step; the
+// y neighbour comes from the adjacent lane.  The parts of a node plane
+// produced by the 4 lane groups (dx, dz) are summed through LDS: dz = 0
+// parts within the layer, dz = 1 parts one layer later.  A finished node
+// plane goes to a coalesced node-major epilogue (residual / damped-Jacobi
+// sweep) whose inputs were requested at the start of the layer.  No
+// workgroup barrier is needed in the walk; load addresses are a scalar base
+// per step plus a per-lane offset fixed for the task.
 //
-// Tiling: a workgroup owns a TX x TY tile of nodes of every node plane in a z
-// chunk and walks the element layers iz = z0-1 .. z1-1 (between node planes
-// iz and iz+1).  Per layer the (TX+1)(TY+1) elements touching the tile are
-// processed in 4 passes by element parity (ex%2, ey%2): elements of one
-// parity share no node, so the scatter into the LDS node accumulators needs
-// no atomics and the summation order is fixed.  Plane iz is complete after
-// layer iz; its epilogue (residual / Jacobi sweep) runs from LDS.  The next
-// node plane and element layer are prefetched into registers during the
-// MFMA passes.  Workgroups are ordered XCD-aware like the FP32/FP64 kernel.
+// Precision: BF16 has the FP32 exponent range, so B0 holds s_gy * u_e(gy)
+// rounded per element (B1, the neighbouring column's B0, then carries
+// s_gy-1).  FP16 columns are scaled by the reciprocal of the element's max
+// |u| (and KE by 1/max|KE|) to stay inside the FP16 range; there the scales
+// s_gy * max|u_e| are applied to the FP32 results of the two products.
 #include "tl_internal.hpp"
 
 #include <algorithm>
 #include <cmath>
-#include <cstring>
+#include <cstdlib>
 #include <type_traits>
 
 namespace tl {
 
 namespace {
 
-constexpr int BS = 256;
-constexpr int TX = 8;    // owned nodes per tile in x
-constexpr int TY = 49;   // owned nodes per tile in y
-constexpr int WX = TX + 2, WY = TY + 2;          // node window (1-node halo)
-constexpr int EX = TX + 1, EY = TY + 1;          // elements touching the tile
-constexpr int NWIN = WX * WY;                    // window nodes per plane
-constexpr int NOWN = TX * TY;                    // owned nodes per plane
-constexpr int NELT = EX * EY;                    // elements per layer
-constexpr int PLANE_VALS = NWIN * 3;
-constexpr int PF_PLANE = (PLANE_VALS + BS - 1) / BS;  // prefetch slots per thread
-constexpr int PF_DENS = (NELT + BS - 1) / BS;
+constexpr int BS = 256;           // 4 wavefronts, one task each
+constexpr int WAVES = BS / 64;
+constexpr int CH = 14;            // nodes produced per y chunk (16 MFMA columns)
+constexpr int TX = 8;             // node lines per task in x
+constexpr int NPL = TX * CH;      // nodes of one node plane of a task
+// LDS per wave: parts [dx][c][NPL] of the current layer (dz = 0) and, by
+// layer parity, of the previous one (dz = 1).  The part stride and the gap
+// before the dz = 1 parts place the four lane groups of one write
+// instruction on disjoint LDS banks.
+constexpr int PART = 352;
+constexpr int HI_OFF = 2 * PART + 32;
+constexpr int WAVE_LDS = 2176;
+static_assert(3 * NPL <= PART && HI_OFF + 4 * PART <= WAVE_LDS, "LDS layout");
+constexpr int EPR = (NPL + 63) / 64;  // epilogue nodes per lane
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
@@ -68,213 +83,272 @@ __device__ inline floatx4 mfma16x16x32(typename Vec8<IN>::type a, typename Vec8<
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
-struct MfmaArgs {
-  const float* u;         // V-cycle iterate (fp32, full DOF vector)
-  const float* sp;        // padded SIMP scale (fp32)
-  const float* KEf;       // element matrix, fp32, 24x24 row-major
-  float kinv, kmax;       // 1/max|KE| (applied before the conversion) and max|KE|
-  const uint8_t* mask;    // per node
-  const void* b;          // right-hand side (fp64 PCG residual)
-  const float* dinv;      // inverse diagonal (fp32)
-  float omega;
-  void* y;                // output: fp32 (residual, Jacobi) or fp64 (Jacobi + dot)
-  double* partials;
-  const int* done;
-  int nelx, nely, nelz, nx, ny, nz, plane, zchunk, ntx, nty;
-};
-
-// local node k of an H8 element: offsets from its (ix,iy,iz) corner node
-__device__ constexpr int odx(int k) { return loc_dx(k); }
-__device__ constexpr int ody(int k) { return loc_dy(k); }
-__device__ constexpr int odz(int k) { return loc_dz(k); }
-
-// element of parity class cls (0..3) with rank r -> local (lex, ley); false
-// if r is past the class size
-__device__ inline bool class_elem(int cls, int r, int& lex, int& ley) {
-  const int px = cls & 1, py = cls >> 1;
-  const int cx = (EX - px + 1) / 2, cy = (EY - py + 1) / 2;
-  if (r >= cx * cy) return false;
-  lex = px + 2 * (r / cy);
-  ley = py + 2 * (r % cy);
-  return true;
+// value of lane m-1 / m+1 within each row of 16 lanes (0 at the row ends)
+__device__ inline float from_below(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               0x111 /* row_shr:1 */, 0xf, 0xf,
+                                                               true));
+}
+__device__ inline float from_above(float v) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v),
+                                                               0x101 /* row_shl:1 */, 0xf, 0xf,
+                                                               true));
 }
 
-template <typename IN, int MODE>
-__global__ __launch_bounds__(BS) void kum3d_kernel(MfmaArgs a) {
-  if (a.done && *a.done) return;
-  __shared__ float planes[2][PLANE_VALS];  // node planes by parity of iz
-  __shared__ float acc[2][NOWN * 3];       // accumulators of owned nodes, by plane parity
-  __shared__ float dens[NELT];             // SIMP scale of the current element layer
-  __shared__ double shred[BS / 64 + 2];
+// element of `base` at a 32-bit byte offset (uniform base + per-lane 32-bit
+// offset: the scalar-base addressing form, no 64-bit per-lane arithmetic)
+template <typename T>
+__device__ inline T ldo(const T* base, unsigned byte_off) {
+  return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+struct F3 {
+  float v[3];
+};
+template <typename T>
+__device__ inline void sto(T* base, unsigned byte_off, T v) {
+  *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
 
-  // XCD-aware order of (tile x, tile y, z chunk): each XCD gets a contiguous range
-  const unsigned nb = gridDim.x;
-  const unsigned hw = blockIdx.x;
-  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
-  const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
-  const int ty = int(logical % a.nty);
-  const int rest = int(logical / a.nty);
-  const int tx = rest % a.ntx;
-  const int tz = rest / a.ntx;
-  const int x0 = tx * TX, y0 = ty * TY;
-  const int z0 = tz * a.zchunk;
-  const int z1 = min(a.nz, z0 + a.zchunk);
+struct ColArgs {
+  const float* u;         // V-cycle iterate (fp32, full DOF vector)
+  const float* sp;        // padded SIMP scale (fp32), (nelz+2) x (nelx+2) x (nely+2)
+  const float* KEf;       // element matrix, fp32, 24x24 row-major
+  float kinv, kmax;       // 1/max|KE| and max|KE|
+  const uint8_t* mask;    // per node, bit c = DOF c fixed
+  const double* b;        // right-hand side (fp64 PCG residual)
+  const float* dinv;      // inverse diagonal (fp32)
+  float omega;
+  void* y;                // fp32 (residual, Jacobi) or fp64 (Jacobi + dot)
+  double* partials;
+  const int* done;
+  int nelx, nely, nelz, nx, ny, nz;
+  int zlen, ntx, nyc, ntask;
+};
+
+template <typename IN, int MODE>
+__global__ __launch_bounds__(BS, 4) void
+kucol_kernel(ColArgs a) {
+  if (a.done && *a.done) return;
+  __shared__ float lds[WAVES * WAVE_LDS];
+  __shared__ double shred[WAVES + 2];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int kq = lane >> 4, col = lane & 15;
+  const int m = lane & 15, kq = lane >> 4;
+  const int dx = kq & 1, dz = kq >> 1;
 
-  // A operands: KE rows (tile 0: 0-15, tile 1: 16-31), K slice 8*kq..8*kq+7, zero padded
-  typename Vec8<IN>::type A0, A1;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) {
-    const int k = 8 * kq + j;
-    const int r0 = col, r1 = 16 + col;
-    A0[j] = IN(k < 24 ? a.KEf[r0 * 24 + k] * a.kinv : 0.f);
-    A1[j] = IN((k < 24 && r1 < 24) ? a.KEf[r1 * 24 + k] * a.kinv : 0.f);
-  }
-
-  // staging helpers ---------------------------------------------------------
-  auto plane_val = [&](int pz, int t) -> float {
-    // window entry t = (wx * WY + wy) * 3 + c
-    const int c = t % 3, wn = t / 3;
-    const int wx = wn / WY, wy = wn % WY;
-    const int gx = x0 - 1 + wx, gy = y0 - 1 + wy;
-    if (pz < 0 || pz >= a.nz || gx < 0 || gx >= a.nx || gy < 0 || gy >= a.ny) return 0.f;
-    return a.u[((int64_t(pz) * a.nx + gx) * a.ny + gy) * 3 + c];
-  };
-  auto dens_val = [&](int ez, int e) -> float {
-    // element e = lex * EY + ley of layer ez; padded array covers ex,ey,ez in [-1, nel]
-    const int lex = e / EY, ley = e % EY;
-    const int ex = x0 - 1 + lex, ey = y0 - 1 + ley;
-    if (ez < -1 || ez > a.nelz || ex > a.nelx || ey > a.nely) return 0.f;
-    return a.sp[(int64_t(ez + 1) * (a.nelx + 2) + ex + 1) * (a.nely + 2) + ey + 1];
-  };
-
-  // prologue: planes z0-1, z0 and layer z0-1; clear accumulators
-  for (int t = threadIdx.x; t < PLANE_VALS; t += BS) {
-    planes[(z0 - 1) & 1][t] = plane_val(z0 - 1, t);
-    planes[z0 & 1][t] = plane_val(z0, t);
-  }
-  for (int e = threadIdx.x; e < NELT; e += BS) dens[e] = dens_val(z0 - 1, e);
-  for (int t = threadIdx.x; t < 2 * NOWN * 3; t += BS) (&acc[0][0])[t] = 0.f;
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs
+  // (each with its own L2); XCD x runs a contiguous range of tasks, so the
+  // node lines and planes shared by neighbouring tasks hit its own L2.
+  const unsigned nb = gridDim.x, hw = blockIdx.x;
+  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
+  // wave-uniform task index (scalar registers; lets the walk branch on SCC)
+  const int task = __builtin_amdgcn_readfirstlane(int(logical) * WAVES + wave);
 
   double dot = 0.0;
-  for (int iz = z0 - 1; iz < z1; ++iz) {
-    __syncthreads();  // planes iz, iz+1, layer iz densities and cleared accs visible
-    // prefetch plane iz+2 and layer iz+1 into registers
-    float pfp[PF_PLANE], pfd[PF_DENS];
+  if (task < a.ntask) {
+    const int cy = task % a.nyc;
+    const int rest = task / a.nyc;
+    const int tx = rest % a.ntx, cz = rest / a.ntx;
+    const int x0 = tx * TX, y0 = cy * CH;
+    const int z0 = cz * a.zlen, z1 = min(a.nz, z0 + a.zlen);
+    const int nxl = min(TX, a.nx - x0);  // node lines of this task
+    const int gy = y0 - 1 + m;
+    float* lw = lds + wave * WAVE_LDS;
+
+    // A operands: lane row m -> node line m>>2 (dx_o, dz_o), component m&3
+    // (3: zero row); K slice kq -> node line (dx_i, dz_i) = (kq&1, kq>>1),
+    // entries j = 3*dy_i + c_i (j >= 6: zero)
+    typename Vec8<IN>::type A0, A1;
+    {
+      const int line = m >> 2, c = m & 3;
+      const int dxo = line & 1, dzo = line >> 1;
 #pragma unroll
-    for (int k = 0; k < PF_PLANE; ++k) {
-      const int t = threadIdx.x + k * BS;
-      pfp[k] = (t < PLANE_VALS) ? plane_val(iz + 2, t) : 0.f;
-    }
-#pragma unroll
-    for (int k = 0; k < PF_DENS; ++k) {
-      const int e = threadIdx.x + k * BS;
-      pfd[k] = (e < NELT) ? dens_val(iz + 1, e) : 0.f;
-    }
-    const float* plo = planes[iz & 1];
-    const float* phi = planes[(iz + 1) & 1];
-    float* acc_lo = acc[iz & 1];        // node plane iz
-    float* acc_hi = acc[(iz + 1) & 1];  // node plane iz+1
-    const bool want_lo = iz >= z0;      // plane iz owned by this chunk
-    const bool want_hi = iz + 1 < z1;   // plane iz+1 owned by this chunk
-    // 4 parity passes of element groups (16 elements per MFMA column block)
-#pragma unroll 1
-    for (int cls = 0; cls < 4; ++cls) {
-      const int px = cls & 1, py = cls >> 1;
-      const int csize = ((EX - px + 1) / 2) * ((EY - py + 1) / 2);
-      const int ngroups = (csize + 15) / 16;
-#pragma unroll 1
-      for (int g = wave; g < ngroups; g += BS / 64) {
-        int lex = 0, ley = 0;
-        const bool valid = class_elem(cls, g * 16 + col, lex, ley);
-        // B operand: this lane's 8 entries of element column `col`
-        float v[8];
-        float m = 0.f;
-#pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const int k = 8 * kq + j;
-          float val = 0.f;
-          if (valid && k < 24) {
-            const int ln = k / 3, c = k % 3;
-            const int wn = (lex + odx(ln)) * WY + ley + ody(ln);
-            val = (odz(ln) ? phi : plo)[wn * 3 + c];
-          }
-          v[j] = val;
-          m = fmaxf(m, fabsf(val));
+      for (int j = 0; j < 8; ++j) {
+        float v0 = 0.f, v1 = 0.f;
+        if (c < 3 && j < 6) {
+          const int col = 3 * loc_of(dx, j / 3, dz) + j % 3;
+          v0 = a.KEf[(3 * loc_of(dxo, 0, dzo) + c) * 24 + col] * a.kinv;
+          v1 = a.KEf[(3 * loc_of(dxo, 1, dzo) + c) * 24 + col] * a.kinv;
         }
-        m = fmaxf(m, __shfl_xor(m, 16, 64));
-        m = fmaxf(m, __shfl_xor(m, 32, 64));
-        const float inv = m > 0.f ? 1.f / m : 0.f;
-        typename Vec8<IN>::type B;
+        A0[j] = IN(v0);
+        A1[j] = IN(v1);
+      }
+    }
+
+    // Per-lane byte offsets, fixed for the task.  Every value read through a
+    // clamped index below is only multiplied by the zero SIMP scale of an
+    // element of the padded border.
+    const int gyn = min(max(gy, 0), a.ny - 1);
+    const int gye = min(gy, a.nely);  // >= -1; nely is the zero border
+    const unsigned lane_u = (__umul24(__umul24(dz, a.nx) + dx, a.ny) + gyn) * 12u;
+    const unsigned lane_s = unsigned(gye + 1) * 4u;
+    // step inputs of element column (x0-1+i, layer L): node (ex+dx, gy, L+dz)
+    // and the scale of element (ex, gy, L); scalar base + lane offset
+    auto load_step = [&](int i, int L, F3& u3, float& sv) {
+      const int ex = x0 - 1 + i;
+      const int exc = min(max(ex, 0), a.nelx - 1), Lz = min(max(L, 0), a.nelz - 1);
+      const unsigned su = unsigned(Lz * a.nx + exc) * unsigned(a.ny) * 12u;
+      u3 = ldo(reinterpret_cast<const F3*>(reinterpret_cast<const char*>(a.u) + su), lane_u);
+      const int exs = min(ex, a.nelx), Ls = min(L, a.nelz);
+      const unsigned ss = unsigned((Ls + 1) * (a.nelx + 2) + exs + 1) * unsigned(a.nely + 2) * 4u;
+      sv = ldo(reinterpret_cast<const float*>(reinterpret_cast<const char*>(a.sp) + ss), lane_s);
+    };
+    // epilogue nodes of this lane: node n = lane + 64k of the task's plane
+    unsigned eo[EPR];
+    bool eok[EPR];
 #pragma unroll
-        for (int j = 0; j < 8; ++j) B[j] = IN(v[j] * inv);
-        floatx4 d0 = {0.f, 0.f, 0.f, 0.f}, d1 = {0.f, 0.f, 0.f, 0.f};
-        d0 = mfma16x16x32<IN>(A0, B, d0);
-        d1 = mfma16x16x32<IN>(A1, B, d1);
-        if (!valid) continue;
-        const float scale = dens[lex * EY + ley] * m * a.kmax;
-        // lane holds rows 4*kq+i (tile 0) and 16+4*kq+i (tile 1) of its column
+    for (int k = 0; k < EPR; ++k) {
+      const int n = lane + 64 * k, nn = min(n, NPL - 1);
+      const int line = nn / CH, node = nn - line * CH;
+      eok[k] = n < NPL && line < nxl && y0 + node < a.ny;
+      eo[k] = __umul24(min(x0 + line, a.nx - 1), a.ny) + min(y0 + node, a.ny - 1);
+    }
+    const unsigned plane = unsigned(a.nx) * unsigned(a.ny);
+
+    F3 cu[TX + 1];
+    float cs[TX + 1];
 #pragma unroll
-        for (int tile = 0; tile < 2; ++tile) {
+    for (int i = 0; i <= TX; ++i) load_step(i, z0 - 1, cu[i], cs[i]);
+
+    for (int ez = z0 - 1; ez < z1; ++ez) {
+      const bool out_plane = ez >= z0;
+      float* lo = lw;                                    // [dx][c][NPL], this layer
+      float* hi_w = lw + HI_OFF + (ez & 1) * 2 * PART;    // dz = 1 parts of this layer
+      const float* hi_r = lw + HI_OFF + ((ez + 1) & 1) * 2 * PART;  // of the previous layer
+      // epilogue inputs of node plane ez (unconditional, clamped; requested
+      // before the walk so the wait counts below are static)
+      const unsigned zoff = unsigned(max(ez, 0)) * plane;
+      double eb[EPR][3];
+      F3 ed[EPR], exv[EPR];
+      unsigned em[EPR];
 #pragma unroll
-          for (int i = 0; i < 4; ++i) {
-            const int row = 16 * tile + 4 * kq + i;
-            if (row >= 24) continue;
-            const int ln = row / 3, c = row % 3;
-            const int wx = lex + odx(ln), wy = ley + ody(ln);
-            if (wx < 1 || wx > TX || wy < 1 || wy > TY) continue;  // not owned
-            const bool hi = odz(ln) != 0;
-            if (hi ? !want_hi : !want_lo) continue;
-            float* dst = hi ? acc_hi : acc_lo;
-            const int o = ((wx - 1) * TY + (wy - 1)) * 3 + c;
-            dst[o] += (tile ? d1[i] : d0[i]) * scale;
-          }
+      for (int k = 0; k < EPR; ++k) {
+        const unsigned nd = zoff + eo[k];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) eb[k][c] = ldo(a.b, nd * 24u + 8u * c);
+        em[k] = ldo(a.mask, nd);
+        if constexpr (MODE != KU_RESID) {
+          ed[k] = ldo(reinterpret_cast<const F3*>(a.dinv), nd * 12u);
+          exv[k] = ldo(reinterpret_cast<const F3*>(a.u), nd * 12u);
         }
       }
-      __syncthreads();  // parity classes never share nodes: order the passes
-    }
-    // epilogue of node plane iz (complete after this layer)
-    if (want_lo) {
-      for (int t = threadIdx.x; t < NOWN * 3; t += BS) {
-        const int c = t % 3, on = t / 3;
-        const int owx = on / TY, owy = on % TY;
-        const int gx = x0 + owx, gy = y0 + owy;
-        if (gx >= a.nx || gy >= a.ny) continue;
-        const int64_t node = (int64_t(iz) * a.nx + gx) * a.ny + gy;
-        const int64_t g = node * 3 + c;
-        const bool fixed = (a.mask[node] >> c) & 1;
-        const float accv = acc_lo[t];
-        const float bb = float(static_cast<const double*>(a.b)[g]);
-        if constexpr (MODE == KU_RESID) {
-          static_cast<float*>(a.y)[g] = fixed ? 0.f : bb - accv;
+#pragma unroll
+      for (int i = 0; i <= TX; ++i) {
+        const F3 u3 = cu[i];
+        const float s0 = cs[i];
+        float up[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) up[c] = from_above(u3.v[c]);
+        const float s1 = from_below(s0);
+        typename Vec8<IN>::type B0;
+        float g0, g1;
+        if constexpr (std::is_same<IN, __bf16>::value) {
+          // bf16: B0 = s_gy * u_e(gy) rounded per element (B1 = B0 of column
+          // m-1 then carries s_gy-1); rounding the iterate once for all its
+          // elements instead gives a ~1.5x larger operator error
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            B0[c] = IN(u3.v[c] * s0);
+            B0[3 + c] = IN(up[c] * s0);
+          }
+          g0 = g1 = a.kmax;
         } else {
-          const float xc = plo[((owx + 1) * WY + owy + 1) * 3 + c];
-          const float vj = fixed ? 0.f : xc + a.omega * a.dinv[g] * (bb - accv);
-          if constexpr (MODE == KU_JACOBI_DOT) {
-            static_cast<double*>(a.y)[g] = double(vj);
-            dot += double(bb) * double(vj);
-          } else {
-            static_cast<float*>(a.y)[g] = vj;
+          // fp16: column scale 1/max|u_e| of element gy (4 node lines x 2 nodes)
+          float am = fmaxf(fabsf(u3.v[0]), fmaxf(fabsf(u3.v[1]), fabsf(u3.v[2])));
+          am = fmaxf(am, __builtin_bit_cast(float, __builtin_amdgcn_ds_swizzle(
+                                                       __builtin_bit_cast(int, am), 0x401F)));
+          am = fmaxf(am, __builtin_bit_cast(float, __builtin_amdgcn_ds_bpermute(
+                                                       (lane ^ 32) << 2, __builtin_bit_cast(int, am))));
+          const float e0 = fmaxf(am, from_above(am));
+          const float inv = e0 > 0.f ? __builtin_amdgcn_rcpf(e0) : 0.f;
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            B0[c] = IN(u3.v[c] * inv);
+            B0[3 + c] = IN(up[c] * inv);
           }
+          g0 = s0 * e0 * a.kmax;
+          g1 = s1 * from_below(e0) * a.kmax;
+        }
+        B0[6] = B0[7] = IN(0.f);
+        // B1 (element gy-1: nodes gy-1, gy) is B0 of column m-1
+        typedef int int4v __attribute__((ext_vector_type(4)));
+        int4v w = __builtin_bit_cast(int4v, B0);
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          w[j] = __builtin_amdgcn_update_dpp(0, w[j], 0x111 /* row_shr:1 */, 0xf, 0xf, true);
+        const typename Vec8<IN>::type B1 = __builtin_bit_cast(typename Vec8<IN>::type, w);
+        // next layer, same step: issued after the last use of this step's
+        // inputs so the loads land in the same registers (no copies, no wait
+        // at the layer boundary); past the last layer they load clamped dummies
+        load_step(i, ez + 1, cu[i], cs[i]);
+        const floatx4 zero = {0.f, 0.f, 0.f, 0.f};
+        float part[3];
+        if constexpr (std::is_same<IN, __bf16>::value) {
+          const floatx4 D = mfma16x16x32<IN>(A1, B1, mfma16x16x32<IN>(A0, B0, zero));
+#pragma unroll
+          for (int c = 0; c < 3; ++c) part[c] = g0 * D[c];
+        } else {
+          const floatx4 D0 = mfma16x16x32<IN>(A0, B0, zero);
+          const floatx4 D1 = mfma16x16x32<IN>(A1, B1, zero);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) part[c] = fmaf(g0, D0[c], g1 * D1[c]);
+        }
+        // lane (kq, m): part (dx, dz) of node (ex+dx, gy, ez+dz) -> line l = i-1+dx
+        const int l = i - 1 + dx;
+        if (m >= 1 && m <= CH && l >= 0 && l < nxl) {
+          float* dst = (dz ? hi_w : lo) + dx * PART + l * CH + (m - 1);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) dst[c * NPL] = part[c];
         }
       }
-    }
-    __syncthreads();  // epilogue read plane iz and acc_lo
-    // recycle: plane iz's buffer receives plane iz+2, acc_lo becomes plane iz+2's
-    float* pdst = planes[iz & 1];
+      if (out_plane) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
-    for (int k = 0; k < PF_PLANE; ++k) {
-      const int t = threadIdx.x + k * BS;
-      if (t < PLANE_VALS) pdst[t] = pfp[k];
-    }
+        for (int k = 0; k < EPR; ++k) {
+          if (!eok[k]) continue;
+          const int n = lane + 64 * k;
+          const unsigned nd = zoff + eo[k];
+          float acc[3];
 #pragma unroll
-    for (int k = 0; k < PF_DENS; ++k) {
-      const int e = threadIdx.x + k * BS;
-      if (e < NELT) dens[e] = pfd[k];
+          for (int c = 0; c < 3; ++c) {
+            const int o = c * NPL + n;
+            // node plane ez: (dx 0 + dx 1) of layer ez + (dx 0 + dx 1) of layer ez-1
+            acc[c] = (lo[o] + lo[PART + o]) + (hi_r[o] + hi_r[PART + o]);
+          }
+          if constexpr (MODE == KU_RESID) {
+            F3 out;
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+              out.v[c] = ((em[k] >> c) & 1u) ? 0.f : float(eb[k][c]) - acc[c];
+            sto(reinterpret_cast<F3*>(a.y), nd * 12u, out);
+          } else {
+            float vj[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              const float bb = float(eb[k][c]);
+              vj[c] = ((em[k] >> c) & 1u) ? 0.f
+                                          : exv[k].v[c] + a.omega * ed[k].v[c] * (bb - acc[c]);
+              if constexpr (MODE == KU_JACOBI_DOT) dot += double(bb) * double(vj[c]);
+            }
+            if constexpr (MODE == KU_JACOBI_DOT) {
+#pragma unroll
+              for (int c = 0; c < 3; ++c) sto(static_cast<double*>(a.y), nd * 24u + 8u * c, double(vj[c]));
+            } else {
+              F3 out;
+#pragma unroll
+              for (int c = 0; c < 3; ++c) out.v[c] = vj[c];
+              sto(reinterpret_cast<F3*>(a.y), nd * 12u, out);
+            }
+          }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
     }
-    for (int t = threadIdx.x; t < NOWN * 3; t += BS) acc_lo[t] = 0.f;
   }
   if constexpr (MODE == KU_JACOBI_DOT) {
     double vv[1] = {dot};
@@ -283,12 +357,24 @@ __global__ __launch_bounds__(BS) void kum3d_kernel(MfmaArgs a) {
   }
 }
 
+int task_zlen(int ntx, int nyc, int nz) {
+  // z range per task: about 4 tasks per SIMD (1024 SIMDs), at least 4 layers
+  if (const char* e = std::getenv("TOPOPTLAB_AMD_MFMA_ZLEN")) {
+    const int v = std::atoi(e);
+    if (v >= 1) return std::min(v, nz);
+  }
+  const int64_t cols = int64_t(ntx) * nyc;
+  const int zl = int((cols * nz + 4095) / 4096);
+  return std::max(4, std::min(zl, nz));
+}
+
 }  // namespace
 
 // Launch the MFMA operator for mode KU_RESID (y fp32), KU_JACOBI (y fp32) or
 // KU_JACOBI_DOT (y fp64; one partial sum of b.y per workgroup in `partials`,
 // *nblocks_out of them, to be folded by the caller).  b is the fp64 PCG
-// residual, dinv the fp32 inverse diagonal.
+// residual, dinv the fp32 inverse diagonal, u the fp32 iterate (zero on fixed
+// DOFs).
 int fine_apply_mfma(tl_problem* P, const void* u, void* y, int prec, int mode, const void* b,
                     const void* dinv, double omega, double* partials, const int* done,
                     unsigned* nblocks_out) {
@@ -297,38 +383,41 @@ int fine_apply_mfma(tl_problem* P, const void* u, void* y, int prec, int mode, c
   TL_ARG(prec == TL_PREC_BF16 || prec == TL_PREC_FP16, "MFMA operator needs BF16 or FP16");
   TL_ARG(P->op.KE32.ptr != nullptr && P->op.sp32.ptr != nullptr && P->op.mask.ptr != nullptr,
          "internal: operator buffers missing");
+  TL_ARG(b != nullptr && (mode == KU_RESID || dinv != nullptr), "internal: MFMA operand missing");
   double kmax = 0.0;
   for (double v : P->KEh) kmax = std::max(kmax, std::fabs(v));
   TL_ARG(kmax > 0.0 && std::isfinite(kmax), "element matrix is zero or not finite");
-  MfmaArgs a{};
+  ColArgs a{};
   a.u = static_cast<const float*>(u);
   a.sp = P->op.sp32.as<float>();
   a.KEf = P->op.KE32.as<float>();
   a.kinv = float(1.0 / kmax);
   a.kmax = float(kmax);
   a.mask = P->op.mask.as<uint8_t>();
-  a.b = b;
+  a.b = static_cast<const double*>(b);
   a.dinv = static_cast<const float*>(dinv);
   a.omega = float(omega);
   a.y = y;
   a.partials = partials;
   a.done = done;
   a.nelx = g.nelx; a.nely = g.nely; a.nelz = g.nelz;
-  a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.plane = g.nx * g.ny;
+  a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
   a.ntx = (g.nx + TX - 1) / TX;
-  a.nty = (g.ny + TY - 1) / TY;
-  // z chunk: about 1024 workgroups (4 per CU), chunks of >= 2 node planes
-  int zc = g.nz;
-  while (zc > 2 && int64_t(a.ntx) * a.nty * ((g.nz + zc - 1) / zc) < 1024) zc = (zc + 1) / 2;
-  a.zchunk = zc;
-  const int64_t nblocks = int64_t(a.ntx) * a.nty * ((g.nz + zc - 1) / zc);
+  a.nyc = (g.ny + CH - 1) / CH;
+  a.zlen = task_zlen(a.ntx, a.nyc, g.nz);
+  const int64_t ntask = int64_t(a.ntx) * a.nyc * ((g.nz + a.zlen - 1) / a.zlen);
+  // 32-bit byte offsets and 24-bit index products in the kernel
+  TL_ARG(ntask < (int64_t(1) << 30) && P->ndof * int64_t(sizeof(double)) < (int64_t(1) << 32) &&
+             P->g.npad() < (int64_t(1) << 24) * 4,
+         "grid too large for the MFMA operator (32-bit offsets)");
+  a.ntask = int(ntask);
+  const int64_t nblocks = (ntask + WAVES - 1) / WAVES;
   TL_ARG(nblocks <= kMaxRedBlocks || mode != KU_JACOBI_DOT,
          "MFMA operator grid exceeds the reduction slot count");
-  TL_ARG(nblocks < (int64_t(1) << 31), "grid too large");
   if (nblocks_out) *nblocks_out = unsigned(nblocks);
   const dim3 grid{unsigned(nblocks), 1u, 1u}, block{unsigned(BS), 1u, 1u};
   const bool bf = prec == TL_PREC_BF16;
-#define TL_KUM(IN, MODE) hipLaunchKernelGGL((kum3d_kernel<IN, MODE>), grid, block, 0, P->stream, a)
+#define TL_KUM(IN, MODE) hipLaunchKernelGGL((kucol_kernel<IN, MODE>), grid, block, 0, P->stream, a)
   switch (mode) {
     case KU_RESID: if (bf) TL_KUM(__bf16, KU_RESID); else TL_KUM(_Float16, KU_RESID); break;
     case KU_JACOBI: if (bf) TL_KUM(__bf16, KU_JACOBI); else TL_KUM(_Float16, KU_JACOBI); break;
