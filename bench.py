@@ -2,13 +2,17 @@
 """Benchmark: one FEM solve of the 192x96x96 H8 cantilever (5,447,811 DOF).
 
 A step = one full solve for a given density: SIMP scaling + diagonal, Galerkin
-GMG hierarchy set-up, FP64-outer PCG with the FP32 V-cycle to relative
-residual 1e-8 (true residual recomputed in FP64 afterwards).  Inputs (density,
-load vector) are resident in HBM before the timed region.  Prints one JSON
-line; extra diagnostics go to stderr.
+GMG hierarchy set-up, FP64-outer PCG to relative residual 1e-8 (true residual
+recomputed in FP64 afterwards).  The timed headline solve is the plain FP64
+GMG-PCG (FP64 V-cycle); the mixed-precision forms (FP32 V-cycle, BF16/FP16
+MFMA fine-level contraction, W-cycle) are timed after it and reported with
+their compliance and displacement errors against an FP64 solve to relres
+1e-12.  Inputs (density, load vector) are resident in HBM before the timed
+region.  Prints one JSON line; extra diagnostics go to stderr.
 
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--levels 5]
-                       [--vcycle fp32|fp64|bf16|fp16] [--no-cpu-baseline]
+                       [--vcycle fp64|fp32|bf16|fp16] [--gamma 1] [--no-variants]
+                       [--no-cpu-baseline]
 """
 from __future__ import annotations
 
@@ -86,15 +90,67 @@ def cpu_baseline(sample=(96, 48, 48), nlevels=4):
     return t, it, ndof
 
 
+def kernel_kinds(dims, vcycle, nlevels):
+    """Launch kinds timed through tl_profile: name, algorithmic bytes and
+    flops per launch, roofline bound and peak."""
+    nelx, nely, nelz = dims
+    nel = nelx * nely * nelz
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    nn1 = (nelx // 2 + 1) * (nely // 2 + 1) * (nelz // 2 + 1)
+    nd1 = 3 * nn1
+    flops_ku = 1152.0 * nel
+    vb = 8 if vcycle == "fp64" else 4  # V-cycle vector bytes (bf16/fp16: fp32 vectors)
+    mfma = vcycle in ("bf16", "fp16")
+    if mfma:
+        vk = f"kucol_kernel {vcycle.upper()}-input MFMA (fp32 accumulate)"
+        vbound, vpeak = "hbm", HBM_PEAK_GBS
+    else:
+        vk = f"ku3d_kernel {vcycle.upper()}"
+        vbound, vpeak = "mfma", FP32_PEAK_TFLOPS if vcycle == "fp32" else FP64_PEAK_TFLOPS
+    sb = 27 * 9 * nn1 * vb  # level-1 block stencil bytes
+    return {
+        0: ("ku3d_kernel FP64 fused PCG product (p = z + beta p; q = K p; p.q)",
+            32 * ndof + 8 * nel, flops_ku, "mfma", FP64_PEAK_TFLOPS),
+        1: (f"{vk} V-cycle residual (r = b - K x)",
+            (2 * vb + 8) * ndof + vb * nel + ndof // 3, flops_ku, vbound, vpeak),
+        2: (f"{vk} V-cycle damped-Jacobi sweep (z in FP64, r.z)",
+            (2 * vb + 16) * ndof + vb * nel + ndof // 3, flops_ku, vbound, vpeak),
+        3: (f"st_apply_kernel {vcycle if not mfma else 'fp32'} level-1 residual (27-point block stencil)",
+            sb + 3 * vb * nd1 + nn1, 2.0 * 243 * nn1, "hbm", HBM_PEAK_GBS),
+        4: (f"st_apply_kernel {vcycle if not mfma else 'fp32'} level-1 damped-Jacobi sweep",
+            sb + 4 * vb * nd1 + nn1, 2.0 * 243 * nn1, "hbm", HBM_PEAK_GBS),
+    }
+
+
+def roofline_entry(kind_def, avg_ms, traffic):
+    name, bytes_alg, flops, bound, peak = kind_def
+    if bound == "mfma":
+        ach = flops / (avg_ms * 1e-3) / 1e12
+        ent = {"bound": "mfma", "kernel": name, "achieved": round(ach, 2), "peak": peak,
+               "unit": "TFLOP/s", "frac": round(ach / peak, 4)}
+    else:
+        ach = bytes_alg / (avg_ms * 1e-3) / 1e9
+        ent = {"bound": "hbm", "kernel": name, "achieved": round(ach, 1), "peak": peak,
+               "unit": "GB/s", "frac": round(ach / peak, 4)}
+    ent.update({"traffic": traffic, "avg_us": round(avg_ms * 1e3, 2),
+                "algorithmic_bytes": bytes_alg, "algorithmic_flops": flops})
+    return ent
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--levels", type=int, default=5)
-    ap.add_argument("--vcycle", default="fp32", choices=["fp32", "fp64", "bf16", "fp16"])
+    ap.add_argument("--vcycle", default="fp64", choices=["fp32", "fp64", "bf16", "fp16"],
+                    help="V-cycle precision of the timed (headline) solve; fp64 = plain FP64 GMG-PCG")
+    ap.add_argument("--gamma", type=int, default=1, help="cycle index (1 V, 2 W) of the timed solve")
+    ap.add_argument("--gamma-from", type=int, default=2)
     ap.add_argument("--nelx", type=int, default=192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the mixed-precision solves and the FP64 reference solution")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (5 launches) so that "
                          "FETCH_SIZE/WRITE_SIZE counters can be scaled to bytes")
@@ -113,6 +169,7 @@ def main():
               "(the 192x96x96 solve runs on a single MI355X); use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
+    import ctypes as C
     import torch
     from topoptlab_amd import _lib
     from topoptlab_amd.elements import lk_linear_elast_3d
@@ -125,81 +182,96 @@ def main():
     nely = nelz = nelx // 2
     dims = (nelx, nely, nelz)
     ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
-    nel = nelx * nely * nelz
     x_h = gaussian_random_density(*dims, seed=0)
     _, f_h, fixed, free, _ = clamped_lineload_3d(nelx, nely, nelz, ndof)
     op = StiffnessOperator(nelx, nely, nelz, KE=lk_linear_elast_3d(), fixed=fixed, device=dev)
     x = torch.as_tensor(x_h, device=dev)
     f = torch.as_tensor(f_h[:, 0], device=dev)
-
-    def step():
-        op.set_density(x)
-        op.setup_mg(nlevels=args.levels, precision=args.vcycle)
-        u, info = op.solve(f, rtol=1e-8, preconditioner="gmg", check_every=16)
-        return u, info
-
-    for _ in range(args.warmup):
-        u, info = step()
-    torch.cuda.synchronize()
     lib = _lib.lib()
-    lib.tl_profile(op._h, 1)
-    infos = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        u, info = step()
-        infos.append(info)
-    torch.cuda.synchronize()
-    t = time.perf_counter() - t0
-    lib.tl_profile(op._h, 0)
-    ms = 1e3 * t / args.steps
-    import ctypes as C
-    prof = {}
-    for kind in (0, 1, 2):
-        tot, cnt = C.c_double(), C.c_int64()
-        _lib.check(lib.tl_profile_read(op._h, kind, C.byref(tot), C.byref(cnt)))
-        prof[kind] = (tot.value, cnt.value)
-    iters = infos[-1]["iterations"]
-    relres = infos[-1]["relres_true"]
+
+    def step(vcycle, gamma, gamma_from, rtol=1e-8):
+        op.set_density(x)
+        op.setup_mg(nlevels=args.levels, precision=vcycle, gamma=gamma, gamma_from=gamma_from)
+        return op.solve(f, rtol=rtol, preconditioner="gmg", check_every=16)
+
+    def timed(vcycle, gamma, gamma_from, steps, warmup, profile=False):
+        for _ in range(warmup):
+            step(vcycle, gamma, gamma_from)
+        torch.cuda.synchronize()
+        if profile:
+            lib.tl_profile(op._h, 1)
+        infos = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            u, info = step(vcycle, gamma, gamma_from)
+            infos.append(info)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        prof = {}
+        if profile:
+            lib.tl_profile(op._h, 0)
+            for kind in range(5):
+                tot, cnt = C.c_double(), C.c_int64()
+                _lib.check(lib.tl_profile_read(op._h, kind, C.byref(tot), C.byref(cnt)))
+                prof[kind] = (tot.value / steps, cnt.value / steps)
+        return 1e3 * t / steps, u, infos[-1], prof
+
+    # headline: the plain solve (FP64 V-cycle unless --vcycle says otherwise)
+    ms, u, info, prof = timed(args.vcycle, args.gamma, args.gamma_from, args.steps, args.warmup,
+                              profile=True)
+    iters, relres = info["iterations"], info["relres_true"]
     compliance = float(torch.dot(f, u).item())
-    log(f"solve: {ms:.2f} ms, {iters} PCG iterations, true relres {relres:.3e}, "
-        f"compliance {compliance:.10e}")
-    vb = 8 if args.vcycle == "fp64" else 4  # V-cycle vector bytes (bf16/fp16: fp32 vectors)
-    mfma = args.vcycle in ("bf16", "fp16")
-    flops_ku = 1152.0 * nel
-    vpeak = {"fp32": FP32_PEAK_TFLOPS, "fp64": FP64_PEAK_TFLOPS}.get(args.vcycle, HBM_PEAK_GBS)
-    vkern = (f"kum3d_kernel {args.vcycle.upper()}-input MFMA" if mfma
-             else f"ku3d_kernel {args.vcycle.upper()}")
-    # (name, algorithmic bytes, peak, bound): the MFMA operator's flops are
-    # far below the matrix cores' peak, so it is priced against HBM
-    kinds = {
-        0: ("ku3d_kernel FP64 fused PCG product (p = z + beta p; q = K p; p.q)",
-            32 * ndof + 8 * nel, FP64_PEAK_TFLOPS, "mfma"),
-        1: (f"{vkern} V-cycle residual (r = b - K x)",
-            (2 * vb + 8) * ndof + vb * nel + (ndof // 3), vpeak, "hbm" if mfma else "mfma"),
-        2: (f"{vkern} V-cycle damped-Jacobi sweep (z in FP64, r.z)",
-            (2 * vb + 16) * ndof + vb * nel + (ndof // 3), vpeak, "hbm" if mfma else "mfma"),
-    }
+    log(f"solve ({args.vcycle} V-cycle, gamma {args.gamma}): {ms:.2f} ms, {iters} PCG iterations, "
+        f"true relres {relres:.3e}, compliance {compliance:.10e}")
+    kinds = kernel_kinds(dims, args.vcycle, args.levels)
     for k, (tot, cnt) in prof.items():
         if cnt:
             avg = tot / cnt
-            log(f"kind {k}: {kinds[k][0]}: {cnt} launches, {tot:.3f} ms, {avg * 1e3:.1f} us/launch, "
-                f"{flops_ku / (avg * 1e-3) / 1e12:.1f} TFLOP/s, "
-                f"{kinds[k][1] / (avg * 1e-3) / 1e9:.0f} GB/s algorithmic")
+            name, b, fl, _, _ = kinds[k]
+            log(f"kind {k}: {name}: {cnt:.0f} launches/solve, {tot:.3f} ms/solve, "
+                f"{avg * 1e3:.1f} us/launch, {fl / (avg * 1e-3) / 1e12:.1f} TFLOP/s, "
+                f"{b / (avg * 1e-3) / 1e9:.0f} GB/s algorithmic")
     dom = max(prof, key=lambda k: prof[k][0])
-    name, bytes_alg, peak, bound = kinds[dom]
     avg_ms = prof[dom][0] / max(prof[dom][1], 1)
-    achieved_tf = flops_ku / (avg_ms * 1e-3) / 1e12
-    achieved_gbs = bytes_alg / (avg_ms * 1e-3) / 1e9
+    vb = 8 if args.vcycle == "fp64" else 4
     traffic = solve_traffic_bytes(dims, args.levels, iters, vb)
     solve_tbps = traffic / (ms * 1e-3) / 1e12
-    log(f"dominant kernel: {name}: {avg_ms * 1e3:.1f} us/launch, {achieved_tf:.1f} TFLOP/s")
-    log(f"solve algorithmic traffic {traffic / 1e9:.2f} GB -> {solve_tbps:.2f} TB/s")
     hbm = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists():
-        tj = json.loads(tfile.read_text())
-        hbm = tj.get(f"kind{dom}_{args.vcycle}_{nelx}")
+        hbm = json.loads(tfile.read_text()).get(f"kind{dom}_{args.vcycle}_{nelx}")
+    roof = roofline_entry(kinds[dom], avg_ms, hbm)
+    log(f"dominant kernel: {roof['kernel']}: {roof['avg_us']} us/launch, "
+        f"{roof['achieved']} {roof['unit']} ({100 * roof['frac']:.1f}% of peak)")
+    log(f"solve algorithmic traffic {traffic / 1e9:.2f} GB -> {solve_tbps:.2f} TB/s")
+
+    # mixed-precision forms against an FP64 reference solution at relres 1e-12
+    variants = None
+    if not args.no_variants:
+        u_ref, inf_ref = step("fp64", 1, 1, rtol=1e-12)
+        c_ref = float(torch.dot(f, u_ref).item())
+        nu = torch.linalg.norm(u_ref).item()
+        variants = {}
+        for name, vc, g, gf in (("fp64_vcycle", "fp64", 1, 1), ("fp32_vcycle", "fp32", 1, 1),
+                                ("bf16_mfma_vcycle", "bf16", 1, 1), ("fp16_mfma_vcycle", "fp16", 1, 1),
+                                ("fp16_mfma_wcycle_l2", "fp16", 2, 2)):
+            vms, vu, vinf, _ = timed(vc, g, gf, args.steps, 1)
+            vc_ = float(torch.dot(f, vu).item())
+            variants[name] = {
+                "ms": round(vms, 3), "pcg_iterations": vinf["iterations"],
+                "relres_true": vinf["relres_true"],
+                "compliance_rel_err": abs(vc_ - c_ref) / abs(c_ref),
+                "u_rel_err": torch.linalg.norm(vu - u_ref).item() / nu,
+                "speedup_vs_fp64_vcycle": None}
+            log(f"variant {name}: {vms:.2f} ms, {vinf['iterations']} its, "
+                f"compliance err {variants[name]['compliance_rel_err']:.2e}, "
+                f"u err {variants[name]['u_rel_err']:.2e}")
+        base = variants["fp64_vcycle"]["ms"]
+        for v in variants.values():
+            v["speedup_vs_fp64_vcycle"] = round(base / v["ms"], 3)
+        variants["reference"] = {"vcycle": "fp64", "rtol": 1e-12,
+                                 "pcg_iterations": inf_ref["iterations"], "compliance": c_ref}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -214,6 +286,8 @@ def main():
                           f"scipy sparse kernels run on 1 of {os.cpu_count()} host cores")}
         log(f"cpu baseline sample: {tc:.2f} s for {ndof_s} DOF ({itc} its)")
 
+    dtype = {"fp32": "f32", "fp64": "f64", "bf16": "bf16-input/f32-accumulate MFMA",
+             "fp16": "f16-input/f32-accumulate MFMA"}[args.vcycle]
     out = {
         "metric": METRIC,
         "value": round(ms, 3),
@@ -225,24 +299,18 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64 outer PCG + " + {"fp32": "f32", "fp64": "f64",
-                                        "bf16": "bf16-input/f32-accumulate MFMA",
-                                        "fp16": "f16-input/f32-accumulate MFMA"}[args.vcycle]
-                 + " V-cycle",
+        "dtype": f"f64 outer PCG + {dtype} V-cycle",
         "data": "synthetic (seeded Gaussian random density field, sigma 4, mean 0.3, seed 0)",
         "config": {"workload": f"cantilever3d H8 {nelx}x{nely}x{nelz} ({ndof} DOF), x=0 face "
-                               f"clamped, unit line load, GMG-PCG to relres 1e-8",
+                               f"clamped, unit line load, GMG-PCG to relres 1e-8, plain "
+                               f"{args.vcycle} V-cycle",
                    "dof": ndof, "levels": args.levels, "vcycle": args.vcycle,
+                   "gamma": args.gamma, "gamma_from": args.gamma_from,
                    "omega": 0.6, "smoothing": "V(1,1) damped Jacobi",
                    "pcg_iterations": iters, "relres_true": relres,
                    "compliance": compliance, "solve_hbm_TBps": round(solve_tbps, 3)},
-        "roofline": ({"bound": "mfma", "kernel": name, "achieved": round(achieved_tf, 2),
-                      "peak": peak, "unit": "TFLOP/s", "frac": round(achieved_tf / peak, 4)}
-                     if bound == "mfma" else
-                     {"bound": "hbm", "kernel": name, "achieved": round(achieved_gbs, 1),
-                      "peak": peak, "unit": "GB/s", "frac": round(achieved_gbs / peak, 4)})
-        | {"traffic": hbm, "avg_us": round(avg_ms * 1e3, 2),
-           "algorithmic_bytes": bytes_alg, "algorithmic_flops": flops_ku},
+        "roofline": roof,
+        "variants": variants,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

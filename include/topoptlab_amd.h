@@ -74,6 +74,8 @@ typedef struct {
   int32_t precision;     /* V-cycle arithmetic: TL_PREC_FP64, TL_PREC_FP32, or
                             TL_PREC_BF16 / TL_PREC_FP16 (3D H8 elasticity: fine-
                             level operator on MFMA, fp32 vectors and coarse levels) */
+  int32_t gamma;         /* cycle index on levels >= gamma_from: 1 (or 0) = V-cycle, 2 = W */
+  int32_t gamma_from;    /* first level (>= 1; 0 means 1) cycled gamma times per visit */
 } tl_mg_opts;
 
 /* Galerkin GMG hierarchy for the current density (multigrid.py:89-223,
@@ -128,11 +130,12 @@ int tl_oc_update(int64_t n, const double* x, const double* dc,
                  double* xnew, double* gt, void* stream);
 
 /* Instrumentation (no counterpart in the reference): while enabled, every
- * launch of the fine-grid operator kernel is bracketed by HIP events on the
- * handle's stream.  kind 0: PCG products (K p, true residual), kind 1:
- * V-cycle residual, kind 2: V-cycle Jacobi sweeps.  tl_profile_read
- * synchronises, returns the summed kernel time and launch count of a kind
- * and clears that kind. */
+ * launch of the fine-grid operator kernel and of the first coarse level's
+ * stencil kernel is bracketed by HIP events on the handle's stream.  kind 0:
+ * PCG products (K p, true residual), kind 1: V-cycle residual, kind 2:
+ * V-cycle Jacobi sweeps, kind 3: level-1 residual, kind 4: level-1 Jacobi
+ * sweeps.  tl_profile_read synchronises, returns the summed kernel time and
+ * launch count of a kind and clears that kind. */
 int tl_profile(tl_problem* p, int32_t enable);
 int tl_profile_read(tl_problem* p, int32_t kind, double* total_ms, int64_t* launches);
 

@@ -109,18 +109,22 @@ class Hierarchy:
             fixed, free = cfix, cfree
         self.Dinv = [1.0 / A.diagonal() for A in self.A]
 
-    def vcycle(self, b, lvl=0):
+    def vcycle(self, b, lvl=0, gamma=1, gamma_from=1):
+        """V(nu1, nu2)-cycle from a zero guess; with gamma > 1 every level
+        >= gamma_from repeats its cycle gamma times (W-cycle for gamma = 2):
+        smooth, correct from a fresh coarse cycle, smooth, from the current x."""
         A, Dinv, w = self.A[lvl], self.Dinv[lvl], self.omega
         if lvl == len(self.A) - 1:
             return spsolve(A.tocsc(), b)
         x = np.zeros_like(b)
-        for _ in range(self.nu1):
-            x = x + w * Dinv * (b - A @ x)
-        r = b - A @ x
-        xc = self.vcycle(self.P[lvl].T @ r, lvl + 1)
-        x = x + self.P[lvl] @ xc
-        for _ in range(self.nu2):
-            x = x + w * Dinv * (b - A @ x)
+        for _ in range(gamma if lvl >= gamma_from else 1):
+            for _ in range(self.nu1):
+                x = x + w * Dinv * (b - A @ x)
+            r = b - A @ x
+            xc = self.vcycle(self.P[lvl].T @ r, lvl + 1, gamma, gamma_from)
+            x = x + self.P[lvl] @ xc
+            for _ in range(self.nu2):
+                x = x + w * Dinv * (b - A @ x)
         return x
 
 

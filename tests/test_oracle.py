@@ -216,6 +216,29 @@ def test_gmg_pcg_matches_direct():
     assert info == 0 and rel(u, ud) < 1e-9
 
 
+def test_wcycle_symmetric_and_stronger():
+    """The W-cycle (gamma = 2 on the coarse levels) has no reference
+    counterpart; it is pinned by what it must satisfy: a symmetric
+    preconditioner (symmetric smoothing around an exact coarsest solve),
+    equal to the V-cycle for gamma = 1, and at least as strong as it."""
+    nelx, nely, nelz = 16, 8, 8
+    x = workloads.gaussian_random_density(nelx, nely, nelz, seed=3)
+    ndof = 3 * 17 * 9 * 9
+    f, fixed, free = fem.clamped_lineload_3d(nelx, nely, nelz, ndof)
+    K = fem.apply_bc(fem.assemble_K(fem.ke_elast_3d(), fem.edofmat_3d(nelx, nely, nelz, 3),
+                                    fem.simp(x), ndof), free).tocsr()
+    h = mg.Hierarchy(K, nelx, nely, nelz, 3, fixed, 4)
+    rng = np.random.default_rng(0)
+    v, w = rng.standard_normal(len(free)), rng.standard_normal(len(free))
+    for gf in (1, 2):
+        mv, mw = h.vcycle(v, gamma=2, gamma_from=gf), h.vcycle(w, gamma=2, gamma_from=gf)
+        assert abs(w @ mv - v @ mw) <= 1e-10 * abs(w @ mv)
+    assert np.array_equal(h.vcycle(v, gamma=1, gamma_from=1), h.vcycle(v))
+    its = [mg.pcg(K, f[free, 0], lambda r, g=g, gf=gf: h.vcycle(r, gamma=g, gamma_from=gf),
+                  rtol=1e-10)[2] for g, gf in ((1, 1), (2, 2), (2, 1))]
+    assert its[0] >= its[1] >= its[2]
+
+
 # ---------------------------------------------------------------- density field
 def test_c2_density_field():
     x = workloads.gaussian_random_density(192, 96, 96, seed=0)

@@ -26,7 +26,8 @@ class GridDesc(C.Structure):
 
 class MgOpts(C.Structure):
     _fields_ = [("nlevels", C.c_int32), ("omega", C.c_double), ("nu1", C.c_int32),
-                ("nu2", C.c_int32), ("precision", C.c_int32)]
+                ("nu2", C.c_int32), ("precision", C.c_int32), ("gamma", C.c_int32),
+                ("gamma_from", C.c_int32)]
 
 
 class SolveOpts(C.Structure):

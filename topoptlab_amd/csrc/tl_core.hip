@@ -685,27 +685,36 @@ int fine_apply_ex(tl_problem* P, const void* u, void* y, int prec, int mode, con
     a.kv32[g] = float(P->op.kv[g]);
   }
   const bool fusep = zfuse_pin != nullptr;
+  // kind 0: products of the PCG loop (K p, true residual); kind 1: V-cycle
+  // residual; kind 2: V-cycle Jacobi sweeps
   ProfEvent ev{};
-  const bool prof = P->prof_on;
-  if (prof) {
-    if (P->prof_free.empty()) {
-      TL_HIP(hipEventCreate(&ev.start));
-      TL_HIP(hipEventCreate(&ev.stop));
-    } else {
-      ev = P->prof_free.back();
-      P->prof_free.pop_back();
-    }
-    // kind 0: products of the PCG loop (K p, true residual); kind 1: V-cycle
-    // residual; kind 2: V-cycle Jacobi sweeps
-    ev.kind = (mode == KU_STORE || mode == KU_STORE_DOT) ? 0 : (mode == KU_RESID ? 1 : 2);
-    TL_HIP(hipEventRecord(ev.start, P->stream));
-  }
+  TL_TRY(prof_begin(P, (mode == KU_STORE || mode == KU_STORE_DOT) ? 0 : (mode == KU_RESID ? 1 : 2),
+                    ev));
   const int rc = fine_apply_dispatch(P, a, prec, mode, fusep, b64, out64);
-  if (prof) {
-    TL_HIP(hipEventRecord(ev.stop, P->stream));
-    P->prof_used.push_back(ev);
-  }
+  TL_TRY(prof_end(P, ev));
   return rc;
+}
+
+int prof_begin(tl_problem* P, int kind, ProfEvent& ev) {
+  ev.kind = -1;
+  if (!P->prof_on) return TL_OK;
+  if (P->prof_free.empty()) {
+    TL_HIP(hipEventCreate(&ev.start));
+    TL_HIP(hipEventCreate(&ev.stop));
+  } else {
+    ev = P->prof_free.back();
+    P->prof_free.pop_back();
+  }
+  ev.kind = kind;
+  TL_HIP(hipEventRecord(ev.start, P->stream));
+  return TL_OK;
+}
+
+int prof_end(tl_problem* P, ProfEvent& ev) {
+  if (ev.kind < 0) return TL_OK;
+  TL_HIP(hipEventRecord(ev.stop, P->stream));
+  P->prof_used.push_back(ev);
+  return TL_OK;
 }
 
 static int fine_apply_dispatch(tl_problem* P, KuArgs& a, int prec, int mode, bool fusep, bool b64,

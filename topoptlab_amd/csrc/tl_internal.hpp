@@ -176,6 +176,7 @@ struct Level {
 struct MgState {
   bool ready = false;
   int nlevels = 0, nu1 = 1, nu2 = 1, prec = TL_PREC_FP32;
+  int gamma = 1, gamma_from = 1;  // cycle index on levels >= gamma_from
   double omega = 0.6;
   std::vector<Level> lv;  // lv[0] is the fine grid (no stencil)
   DevBuf G1;              // 8 x (nd*nd) fp64: child Galerkin matrices
@@ -229,6 +230,9 @@ int fine_apply_ex(tl_problem* P, const void* u, void* y, int prec, int mode, con
                   void* pout, const double* beta, double* dot_out, int fin, const int* done,
                   bool out64);
 int fine_diag(tl_problem* P);
+// launch timing (tl_profile): no-ops while profiling is off
+int prof_begin(tl_problem* P, int kind, ProfEvent& ev);
+int prof_end(tl_problem* P, ProfEvent& ev);
 int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done);
 // V-cycle fine operator on MFMA (3D H8 elasticity, prec TL_PREC_BF16/FP16)
 int fine_apply_mfma(tl_problem* P, const void* u, void* y, int prec, int mode, const void* b,

@@ -534,6 +534,10 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
   const T* dinv = std::is_same<T, double>::value ? static_cast<const T*>(L.dinv64.ptr)
                                                   : static_cast<const T*>(L.dinv32.ptr);
   const unsigned nb = unsigned((L.g.nnodes + BS - 1) / BS);
+  // launch timing of the first coarse level: kind 3 residual, kind 4 Jacobi
+  ProfEvent ev{};
+  ev.kind = -1;
+  if (l == 1) TL_TRY(prof_begin(P, 3 + mode, ev));
   if (mode == 0)
     hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 0>), dim3(nb), dim3(BS), 0, P->stream, to_d(L.g),
                        st, x, b, dinv, T(P->mg.omega), L.mask.as<uint8_t>(), y, done);
@@ -541,7 +545,7 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
     hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1>), dim3(nb), dim3(BS), 0, P->stream, to_d(L.g),
                        st, x, b, dinv, T(P->mg.omega), L.mask.as<uint8_t>(), y, done);
   TL_CHECK_LAUNCH();
-  return TL_OK;
+  return prof_end(P, ev);
 }
 
 // Solve A_l x_l = b_l approximately (x_l := V-cycle(b_l), zero initial guess).
@@ -566,29 +570,40 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
   const T* b = vp<T>(L.vb);
   const T* dinv = std::is_same<T, double>::value ? static_cast<const T*>(L.dinv64.ptr)
                                                   : static_cast<const T*>(L.dinv32.ptr);
-  hipLaunchKernelGGL((jacobi_zero_kernel<T, T>), dim3(gsz(n)), dim3(BS), 0, P->stream, b, dinv,
-                     T(M.omega), x, n, done);
-  TL_CHECK_LAUNCH();
-  for (int s = 1; s < M.nu1; ++s) {
-    TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
-    std::swap(L.vx.ptr, L.vt.ptr);
-    x = vp<T>(L.vx); t = vp<T>(L.vt);
-  }
-  TL_TRY((st_apply<NDPN, T>(P, l, 0, x, b, vp<T>(L.vr), done)));
-  Level& C = M.lv[l + 1];
-  hipLaunchKernelGGL((restrict_kernel<NDPN, T>), dim3(unsigned((C.g.nnodes + BS - 1) / BS)),
-                     dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(L.vr),
-                     C.mask.as<uint8_t>(), vp<T>(C.vb), done);
-  TL_CHECK_LAUNCH();
-  TL_TRY((vcycle_level<NDPN, T>(P, l + 1, done)));
-  hipLaunchKernelGGL((prolong_add_kernel<NDPN, T>), dim3(unsigned((L.g.nnodes + BS - 1) / BS)),
-                     dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(C.vx),
-                     L.mask.as<uint8_t>(), x, done);
-  TL_CHECK_LAUNCH();
-  for (int s = 0; s < M.nu2; ++s) {
-    TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
-    std::swap(L.vx.ptr, L.vt.ptr);
-    x = vp<T>(L.vx); t = vp<T>(L.vt);
+  // gamma > 1 (W-cycle): the level's cycle repeats from the current x; each
+  // repetition smooths, corrects from a fresh coarse cycle and smooths again
+  const int gam = l >= M.gamma_from ? M.gamma : 1;
+  for (int g = 0; g < gam; ++g) {
+    if (g == 0) {
+      hipLaunchKernelGGL((jacobi_zero_kernel<T, T>), dim3(gsz(n)), dim3(BS), 0, P->stream, b, dinv,
+                         T(M.omega), x, n, done);
+      TL_CHECK_LAUNCH();
+    } else {
+      TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
+      std::swap(L.vx.ptr, L.vt.ptr);
+      x = vp<T>(L.vx); t = vp<T>(L.vt);
+    }
+    for (int s = 1; s < M.nu1; ++s) {
+      TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
+      std::swap(L.vx.ptr, L.vt.ptr);
+      x = vp<T>(L.vx); t = vp<T>(L.vt);
+    }
+    TL_TRY((st_apply<NDPN, T>(P, l, 0, x, b, vp<T>(L.vr), done)));
+    Level& C = M.lv[l + 1];
+    hipLaunchKernelGGL((restrict_kernel<NDPN, T>), dim3(unsigned((C.g.nnodes + BS - 1) / BS)),
+                       dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(L.vr),
+                       C.mask.as<uint8_t>(), vp<T>(C.vb), done);
+    TL_CHECK_LAUNCH();
+    TL_TRY((vcycle_level<NDPN, T>(P, l + 1, done)));
+    hipLaunchKernelGGL((prolong_add_kernel<NDPN, T>), dim3(unsigned((L.g.nnodes + BS - 1) / BS)),
+                       dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(C.vx),
+                       L.mask.as<uint8_t>(), x, done);
+    TL_CHECK_LAUNCH();
+    for (int s = 0; s < M.nu2; ++s) {
+      TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
+      std::swap(L.vx.ptr, L.vt.ptr);
+      x = vp<T>(L.vx); t = vp<T>(L.vt);
+    }
   }
   (void)nb;
   return TL_OK;
@@ -681,7 +696,10 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
   MgState& M = P->mg;
   const int ndpn = P->ndpn, nd = P->nd;
   const bool rebuild = M.nlevels != L || M.lv.empty();
+  // (zero-initialised gamma fields mean a V-cycle)
+  TL_ARG(o->gamma >= 0 && o->gamma <= 3 && o->gamma_from >= 0, "gamma must be 0..3, gamma_from >= 0");
   M.nu1 = o->nu1; M.nu2 = o->nu2; M.omega = o->omega; M.prec = o->precision;
+  M.gamma = std::max(1, int(o->gamma)); M.gamma_from = std::max(1, int(o->gamma_from));
   hipStream_t s = P->stream;
   if (rebuild) {
     std::vector<std::vector<uint8_t>> masks;

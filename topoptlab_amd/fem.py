@@ -149,13 +149,15 @@ class StiffnessOperator:
 
     # -- multigrid ------------------------------------------------------------
     def setup_mg(self, nlevels: Optional[int] = None, omega: float = 0.6, nu1: int = 1,
-                 nu2: int = 1, precision: str = "fp32", max_coarse_dof: int = 4000) -> int:
+                 nu2: int = 1, precision: str = "fp32", max_coarse_dof: int = 4000,
+                 gamma: int = 1, gamma_from: int = 1) -> int:
         if nlevels is None:
             nlevels = auto_levels(self.nelx, self.nely, self.nelz, self.ndpn, max_coarse_dof)
-        opts = _lib.MgOpts(int(nlevels), float(omega), int(nu1), int(nu2), PRECISIONS[precision])
+        opts = _lib.MgOpts(int(nlevels), float(omega), int(nu1), int(nu2), PRECISIONS[precision],
+                           int(gamma), int(gamma_from))
         self._sync_stream()
         check(lib().tl_setup_mg(self._h, C.byref(opts)), "tl_setup_mg")
-        self._mg_key = (nlevels, omega, nu1, nu2, precision)
+        self._mg_key = (nlevels, omega, nu1, nu2, precision, gamma, gamma_from)
         return nlevels
 
     # -- solve ------------------------------------------------------------------
