@@ -64,7 +64,7 @@ def solve_traffic_bytes(dims, nlevels, iters, vprec_bytes=4):
         if lvl == nlevels - 1:
             coarse += 8 * nd * nd + 2 * w * nd
         else:
-            st = 27 * ndpn * ndpn * nn * w
+            st = 14 * ndpn * ndpn * nn * w  # forward half of the block stencil
             coarse += 2 * st + 12 * w * nd
     return iters * (fine_pcg + fine_v + coarse)
 
@@ -107,7 +107,7 @@ def kernel_kinds(dims, vcycle, nlevels):
     else:
         vk = f"ku3d_kernel {vcycle.upper()}"
         vbound, vpeak = "mfma", FP32_PEAK_TFLOPS if vcycle == "fp32" else FP64_PEAK_TFLOPS
-    sb = 27 * 9 * nn1 * vb  # level-1 block stencil bytes
+    sb = 14 * 9 * nn1 * vb  # level-1 block stencil bytes (forward half: centre + 13 blocks)
     return {
         0: ("ku3d_kernel FP64 fused PCG product (p = z + beta p; q = K p; p.q)",
             32 * ndof + 8 * nel, flops_ku, "mfma", FP64_PEAK_TFLOPS),
@@ -115,7 +115,8 @@ def kernel_kinds(dims, vcycle, nlevels):
             (2 * vb + 8) * ndof + vb * nel + ndof // 3, flops_ku, vbound, vpeak),
         2: (f"{vk} V-cycle damped-Jacobi sweep (z in FP64, r.z)",
             (2 * vb + 16) * ndof + vb * nel + ndof // 3, flops_ku, vbound, vpeak),
-        3: (f"st_apply_kernel {vcycle if not mfma else 'fp32'} level-1 residual (27-point block stencil)",
+        3: (f"st_apply_kernel {vcycle if not mfma else 'fp32'} level-1 residual (27-point block "
+            f"stencil, symmetric half stored)",
             sb + 3 * vb * nd1 + nn1, 2.0 * 243 * nn1, "hbm", HBM_PEAK_GBS),
         4: (f"st_apply_kernel {vcycle if not mfma else 'fp32'} level-1 damped-Jacobi sweep",
             sb + 4 * vb * nd1 + nn1, 2.0 * 243 * nn1, "hbm", HBM_PEAK_GBS),

@@ -167,7 +167,8 @@ struct Level {
   Grid g;
   int64_t ndof = 0;
   DevBuf mask;            // uint8 per node, bit d = DOF d fixed
-  DevBuf st64, st32;      // coarse stencil [ns*ndpn*ndpn][nnodes]
+  DevBuf st64;            // coarse stencil [ns*ndpn*ndpn][nnodes] (fp64, all blocks)
+  DevBuf sth64, st32;     // forward half [(ns/2+1)*ndpn*ndpn][nnodes] (fp64, fp32) for the V-cycle
   DevBuf dinv64, dinv32;  // 1/diag (0 on fixed DOFs)
   DevBuf em;              // element matrices (levels >= 1, fp64)
   DevBuf vb, vx, vr, vt;  // V-cycle work vectors (V-cycle precision)

@@ -165,16 +165,24 @@ __global__ void st_coarsen_kernel(GridD gf, GridD gc, int ndpn, const double* __
   }
 }
 
-// fp32 copy of the stencil and inverse diagonal
+// Forward half of the stencil (centre block and the blocks of neighbours
+// with a positive offset in (z, x, y) order), fp64 and fp32, and the inverse
+// diagonal.  The operator is symmetric, A(i, i-r) = A(i-r, i)^T, so the V-cycle
+// reads a backward block from the neighbour's forward half, transposed: half
+// the stencil bytes, and the applied operator is exactly symmetric.
 __global__ void st_finish_kernel(GridD g, int ndpn, const double* __restrict__ st64,
-                                 float* __restrict__ st32, const uint8_t* __restrict__ mask,
-                                 double* __restrict__ dinv64, float* __restrict__ dinv32) {
-  const int nq = nstencil(g.dim) * ndpn * ndpn;
-  const int64_t tot = g.nnodes * nq;
-  for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
-       t += int64_t(gridDim.x) * BS)
-    st32[t] = float(st64[t]);
+                                 double* __restrict__ sth64, float* __restrict__ st32,
+                                 const uint8_t* __restrict__ mask, double* __restrict__ dinv64,
+                                 float* __restrict__ dinv32) {
   const int kc = center_k(g.dim);
+  const int nqh = (kc + 1) * ndpn * ndpn;
+  const int64_t tot = g.nnodes * nqh;
+  for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
+       t += int64_t(gridDim.x) * BS) {
+    const double v = st64[t + int64_t(kc) * ndpn * ndpn * g.nnodes];
+    sth64[t] = v;
+    st32[t] = float(v);
+  }
   for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < g.nnodes * ndpn;
        t += int64_t(gridDim.x) * BS) {
     const int64_t n = t / ndpn;
@@ -190,16 +198,24 @@ __global__ void st_finish_kernel(GridD g, int ndpn, const double* __restrict__ s
 // ---------------------------------------------------------------------------
 // V-cycle kernels (T = float or double)
 // ---------------------------------------------------------------------------
-// y = b - A x  (MODE 0)   or   y = x + omega*dinv*(b - A x)  (MODE 1)
+// y = b - A x  (MODE 0)   or   y = x + omega*dinv*(b - A x)  (MODE 1), with A
+// from its forward half sth ([kk][d][c][node], kk = k - centre): forward
+// blocks of node i, backward blocks as the transposed forward blocks of the
+// neighbours.  Workgroups take contiguous node ranges per XCD so that the
+// neighbours' blocks (up to one node plane back) are re-read from the XCD's
+// own L2.
 template <int NDPN, typename T, int MODE>
-__global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restrict__ st,
+__global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restrict__ sth,
                                                       const T* __restrict__ x,
                                                       const T* __restrict__ b,
                                                       const T* __restrict__ dinv, T omega,
                                                       const uint8_t* __restrict__ mask,
                                                       T* __restrict__ y, const int* done) {
   if (*done) return;
-  const int64_t n = blockIdx.x * int64_t(BS) + threadIdx.x;
+  const unsigned nb = gridDim.x, hw = blockIdx.x;
+  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
+  const int64_t n = int64_t(logical) * BS + threadIdx.x;
   if (n >= g.nnodes) return;
   int ix, iy, iz;
   node_xyz(g, n, ix, iy, iz);
@@ -207,24 +223,37 @@ __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restri
 #pragma unroll
   for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
   const bool d3 = g.dim == 3;
+  const int kc = center_k(g.dim);
   for (int rz = (d3 ? -1 : 0); rz <= (d3 ? 1 : 0); ++rz)
 #pragma unroll
     for (int rx = -1; rx <= 1; ++rx)
 #pragma unroll
       for (int ry = -1; ry <= 1; ++ry) {
         const int k = kidx(g.dim, rx, ry, rz);
-        // neighbours outside the grid carry zero stencil weight: clamp the read
+        // neighbours outside the grid carry zero weight: clamp the reads
         const int mx = min(max(ix + rx, 0), g.nx - 1), my = min(max(iy + ry, 0), g.ny - 1),
                   mz = min(max(iz + rz, 0), g.nz - 1);
         const int64_t m = node_id(g, mx, my, mz);
         T xm[NDPN];
 #pragma unroll
         for (int c = 0; c < NDPN; ++c) xm[c] = x[m * NDPN + c];
+        if (k >= kc) {  // forward block of node n (zero towards missing neighbours)
+          const T* h = sth + int64_t(k - kc) * NDPN * NDPN * g.nnodes + n;
 #pragma unroll
-        for (int d = 0; d < NDPN; ++d)
+          for (int d = 0; d < NDPN; ++d)
 #pragma unroll
-          for (int c = 0; c < NDPN; ++c)
-            acc[d] += st[int64_t((k * NDPN + d) * NDPN + c) * g.nnodes + n] * xm[c];
+            for (int c = 0; c < NDPN; ++c) acc[d] += h[int64_t(d * NDPN + c) * g.nnodes] * xm[c];
+        } else {  // A(n, m) = A(m, n)^T: forward block (offset -r) of neighbour m
+          const bool ok = node_in(g, ix + rx, iy + ry, iz + rz);
+          const T* h = sth + int64_t(2 * kc - k - kc) * NDPN * NDPN * g.nnodes + m;
+#pragma unroll
+          for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+            for (int c = 0; c < NDPN; ++c) {
+              const T w = h[int64_t(c * NDPN + d) * g.nnodes];
+              acc[d] += (ok ? w : T(0)) * xm[c];
+            }
+        }
       }
   const uint8_t mk = mask[n];
 #pragma unroll
@@ -527,9 +556,9 @@ template <int NDPN, typename T>
 int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const int* done) {
   Level& L = P->mg.lv[l];
   TL_ARG(l >= 1 && l < P->mg.nlevels, "internal: stencil level out of range");
-  TL_NONNULL(L.st64.ptr); TL_NONNULL(L.st32.ptr); TL_NONNULL(L.dinv64.ptr);
+  TL_NONNULL(L.sth64.ptr); TL_NONNULL(L.st32.ptr); TL_NONNULL(L.dinv64.ptr);
   TL_NONNULL(L.dinv32.ptr); TL_NONNULL(L.mask.ptr); TL_NONNULL(x); TL_NONNULL(b); TL_NONNULL(y);
-  const T* st = std::is_same<T, double>::value ? static_cast<const T*>(L.st64.ptr)
+  const T* st = std::is_same<T, double>::value ? static_cast<const T*>(L.sth64.ptr)
                                                 : static_cast<const T*>(L.st32.ptr);
   const T* dinv = std::is_same<T, double>::value ? static_cast<const T*>(L.dinv64.ptr)
                                                   : static_cast<const T*>(L.dinv32.ptr);
@@ -720,7 +749,9 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
       TL_HIP(hipMemcpy(lv.mask.ptr, masks[l].data(), gl.nnodes, hipMemcpyHostToDevice));
       const size_t nst = size_t(nstencil(gl.dim)) * ndpn * ndpn * gl.nnodes;
       TL_TRY(lv.st64.alloc(sizeof(double) * nst));
-      TL_TRY(lv.st32.alloc(sizeof(float) * nst));
+      const size_t nsh = size_t(center_k(gl.dim) + 1) * ndpn * ndpn * gl.nnodes;
+      TL_TRY(lv.sth64.alloc(sizeof(double) * nsh));
+      TL_TRY(lv.st32.alloc(sizeof(float) * nsh));
       TL_TRY(lv.dinv64.alloc(sizeof(double) * lv.ndof));
       TL_TRY(lv.dinv32.alloc(sizeof(float) * lv.ndof));
     }
@@ -777,8 +808,8 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
     Level& lv = M.lv[l];
     const int64_t tot = lv.g.nnodes * nstencil(g.dim) * ndpn * ndpn;
     hipLaunchKernelGGL(st_finish_kernel, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lv.g), ndpn,
-                       lv.st64.as<double>(), lv.st32.as<float>(), lv.mask.as<uint8_t>(),
-                       lv.dinv64.as<double>(), lv.dinv32.as<float>());
+                       lv.st64.as<double>(), lv.sth64.as<double>(), lv.st32.as<float>(),
+                       lv.mask.as<uint8_t>(), lv.dinv64.as<double>(), lv.dinv32.as<float>());
     TL_CHECK_LAUNCH();
   }
   // coarsest dense inverse
