@@ -456,8 +456,11 @@ __device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t
   const int64_t node = int64_t(iz) * a.plane + (active ? j : 0);
   EpiPrefetch<NDPN, T, MODE, TB> ep;
   fetch_epi<NDPN, T, MODE, TB>(a, node, ep);
+  // scales of element layer iz+1 for the next step; on the last node plane
+  // (iz = nelz) there is no next step, and layer nelz+1 would lie past the
+  // padded array: read the (zero) border layer nelz instead
   T Sn[4];
-  load_dens<T>(a, ix, iy, iz + 1, Sn);
+  load_dens<T>(a, ix, iy, min(iz + 1, a.nelz), Sn);
   T acc_top[NDPN], acc_bot[NDPN];
   layer_sum<NDPN, T, SA, SB, FAST, true, true>(a, U, S, acc_top, acc_bot);
   if (active) {
