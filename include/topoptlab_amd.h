@@ -139,6 +139,58 @@ int tl_oc_update(int64_t n, const double* x, const double* dc,
 int tl_profile(tl_problem* p, int32_t enable);
 int tl_profile_read(tl_problem* p, int32_t kind, double* total_ms, int64_t* launches);
 
+/* ---- Learned coarse-grid correction (no counterpart in topoptlab). ----
+ * Layers of a small 3D convolutional network on node grids.  Activations:
+ * fp32, channel-major [c][z][x][y]; weights: fp16 [cout][cin][27], tap
+ * t = (dz+1)*9 + (dx+1)*3 + (dy+1); fp32 accumulation; zero padding.
+ * Stride 2 maps 2m+1 nodes to m+1 per axis.  nz, nx, ny are the input
+ * grid's node counts; all pointers are device pointers on `stream`.
+ * out[co] = relu?(bias[co] + sum_ci w[co][ci] * in[ci]), 1 <= cin <= 64. */
+int tl_nn_conv3d(int32_t cin, int32_t cout, int32_t nz, int32_t nx, int32_t ny, int32_t stride,
+                 int32_t relu, const float* in, const uint16_t* w16, const float* bias,
+                 float* out, void* stream);
+/* gin (+)= d(sum gout * conv)/d in, for gout already masked by the ReLU. */
+int tl_nn_conv3d_bwd_data(int32_t cin, int32_t cout, int32_t nz, int32_t nx, int32_t ny,
+                          int32_t stride, const float* gout, const uint16_t* w16, float* gin,
+                          int32_t accumulate, void* stream);
+/* gw[co][ci][t], gb[co]: weight and bias gradients (fixed-order sums). */
+int tl_nn_conv3d_bwd_weight(int32_t cin, int32_t cout, int32_t nz, int32_t nx, int32_t ny,
+                            int32_t stride, const float* in, const float* gout, float* gw,
+                            float* gb, void* stream);
+/* g[i] = 0 where y[i] <= 0 (ReLU backward on the layer output). */
+int tl_nn_relu_mask(int64_t n, const float* y, float* g, void* stream);
+/* Trilinear node interpolation from a (cz, cx, cy) grid to (2cz-1, 2cx-1,
+ * 2cy-1), channel-wise, and its transpose; accumulate: add into the output. */
+int tl_nn_upsample(int32_t c, int32_t cz, int32_t cx, int32_t cy, const float* coarse,
+                   float* fine, int32_t accumulate, void* stream);
+int tl_nn_upsample_t(int32_t c, int32_t cz, int32_t cx, int32_t cy, const float* gfine,
+                     float* gcoarse, int32_t accumulate, void* stream);
+/* Mean squared error: grad = 2 (pred - target)/n, *loss (device double) =
+ * mean (pred - target)^2; partials: 1024 doubles of scratch. */
+int tl_nn_mse(int64_t n, const float* pred, const float* target, float* grad,
+              double* partials, double* loss, void* stream);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) on fp32 master weights w with
+ * moments m, v; w16 (may be NULL) receives the fp16 copy; step >= 1. */
+int tl_nn_adam(int64_t n, float* w, const float* g, float* m, float* v, uint16_t* w16,
+               int32_t step, float lr, void* stream);
+
+/* Inputs of the network on the level-1 node grid of a 3D H8 problem (even
+ * element counts), channel-major [6][nz1][nx1][ny1]: mean element density
+ * at the node, the load restricted by full weighting times fscale (3),
+ * fixed-node indicator, ones.  xPhys: nel, f: ndof (device, float64). */
+int tl_nn_features(tl_problem* p, const double* xPhys, const double* f, double fscale,
+                   float* out);
+/* out[c][node1] = scale * u at the coinciding fine node (training targets). */
+int tl_nn_inject(tl_problem* p, const double* u, double scale, float* out);
+/* x (ndof, float64) = trilinear prolongation of a level-1 field y[3][...]
+ * (channel-major), zero on fixed DOFs. */
+int tl_nn_prolong(tl_problem* p, const float* y, double* x);
+/* Energy-optimal scaling of an initial guess: x *= alpha with
+ * alpha = (f.x)/(x.K x), which minimises the energy norm of the error over
+ * the line through x; the result is never worse than the zero guess.
+ * work: ndof float64; *alpha returned on the host (synchronises). */
+int tl_energy_scale(tl_problem* p, const double* f, double* x, double* work, double* alpha);
+
 /* mean(a) and max|a-b| of device vectors (topology_optimization.py:516,654). */
 int tl_mean_maxdiff(int64_t n, const double* a, const double* b, double* mean,
                     double* maxdiff, void* stream);

@@ -1,0 +1,168 @@
+"""GPU parity of the learned coarse-correction network (marked ``gpu``):
+every kernel against the numpy oracle (oracle/nn.py), the whole network's
+parameter gradient against the oracle's, Adam, and a test of what the
+trained network gains over the plain GMG-PCG solve.
+
+Tolerances: fp32 accumulation over up to 64*27 products per output, fp16
+weights shared by both sides (the oracle takes the fp16-rounded weights):
+1e-5 relative for single kernels, 1e-4 for the network gradient."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import nn
+from tests.conftest import rel
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _t(a, dev, dtype=None):
+    import torch
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=dtype or torch.float32, device=dev)
+
+
+@pytest.mark.parametrize("stride,relu,shape,cin,cout", [(1, True, (5, 7, 9), 6, 16),
+                                                         (2, True, (5, 7, 9), 16, 32),
+                                                         (1, False, (3, 5, 5), 16, 3),
+                                                         (1, True, (3, 3, 5), 64, 64),
+                                                         (2, False, (7, 5, 3), 3, 9)])
+def test_conv_kernels_match_oracle(gpu, stride, relu, shape, cin, cout):
+    import ctypes as C
+    import torch
+    from topoptlab_amd._lib import check, lib
+    say("conv", stride, relu, shape, cin, cout)
+    rng = np.random.default_rng(cin + cout + stride)
+    a = rng.standard_normal((cin,) + shape).astype(np.float32)
+    w16 = (rng.standard_normal((cout, cin, 27)) * 0.2).astype(np.float16)
+    b = rng.standard_normal(cout).astype(np.float32)
+    w = w16.astype(np.float64)
+    y_ref = nn.conv3d(a.astype(np.float64), w, b.astype(np.float64), stride, relu)
+    da, dw16, db = _t(a, gpu), _t(w16, gpu, torch.float16), _t(b, gpu)
+    y = torch.empty(y_ref.shape, dtype=torch.float32, device=gpu)
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr())
+    check(lib().tl_nn_conv3d(cin, cout, *shape, stride, int(relu), p(da), p(dw16), p(db), p(y), s))
+    assert rel(y.cpu().numpy(), y_ref) < 1e-5
+    gy = rng.standard_normal(y_ref.shape).astype(np.float32)
+    ga_ref, gw_ref, gb_ref = nn.conv3d_backward(a.astype(np.float64), w, gy.astype(np.float64),
+                                                stride, y_ref if relu else None)
+    dgy = _t(gy, gpu)
+    if relu:
+        check(lib().tl_nn_relu_mask(dgy.numel(), p(y), p(dgy), s))
+    ga = torch.full((cin,) + shape, 7.0, dtype=torch.float32, device=gpu)
+    check(lib().tl_nn_conv3d_bwd_data(cin, cout, *shape, stride, p(dgy), p(dw16), p(ga), 0, s))
+    assert rel(ga.cpu().numpy(), ga_ref) < 1e-5
+    # accumulate adds onto what is there
+    check(lib().tl_nn_conv3d_bwd_data(cin, cout, *shape, stride, p(dgy), p(dw16), p(ga), 1, s))
+    assert rel(ga.cpu().numpy(), 2 * ga_ref) < 1e-5
+    gw = torch.empty((cout, cin, 27), dtype=torch.float32, device=gpu)
+    gb = torch.empty(cout, dtype=torch.float32, device=gpu)
+    check(lib().tl_nn_conv3d_bwd_weight(cin, cout, *shape, stride, p(da), p(dgy), p(gw), p(gb), s))
+    assert rel(gw.cpu().numpy(), gw_ref) < 1e-5
+    assert rel(gb.cpu().numpy(), gb_ref) < 1e-5
+
+
+def test_upsample_kernels_match_oracle(gpu):
+    import ctypes as C
+    import torch
+    from topoptlab_amd._lib import check, lib
+    rng = np.random.default_rng(5)
+    a = rng.standard_normal((4, 3, 4, 5)).astype(np.float32)
+    g = rng.standard_normal((4, 5, 7, 9)).astype(np.float32)
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr())
+    out = torch.empty((4, 5, 7, 9), dtype=torch.float32, device=gpu)
+    check(lib().tl_nn_upsample(4, 3, 4, 5, p(_t(a, gpu)), p(out), 0, s))
+    assert rel(out.cpu().numpy(), nn.upsample(a.astype(np.float64))) < 1e-6
+    gc = torch.empty((4, 3, 4, 5), dtype=torch.float32, device=gpu)
+    check(lib().tl_nn_upsample_t(4, 3, 4, 5, p(_t(g, gpu)), p(gc), 0, s))
+    assert rel(gc.cpu().numpy(), nn.upsample_t(g.astype(np.float64))) < 1e-6
+
+
+def test_network_gradient_matches_oracle(gpu):
+    import torch
+    from topoptlab_amd.learned import CoarseNet
+    say("network gradient")
+    net = CoarseNet(gpu, seed=1)
+    rng = np.random.default_rng(2)
+    X0 = rng.standard_normal((6, 5, 5, 9)).astype(np.float32)
+    Y = net.forward(_t(X0, gpu)).clone()
+    gY = rng.standard_normal(Y.shape).astype(np.float32)
+    net.grad.zero_()
+    net.backward(_t(gY, gpu))
+    # the oracle runs on the same parameters: fp16-rounded weights, fp32 biases
+    params = net.w.cpu().numpy().astype(np.float64)
+    for L in nn.LAYERS:
+        o, nw, _ = net.offsets[L[0]]
+        params[o:o + nw] = net.w16[o:o + nw].cpu().numpy().astype(np.float64)
+    Y_ref, A = nn.net_forward(params, X0.astype(np.float64))
+    assert rel(Y.cpu().numpy(), Y_ref) < 1e-4
+    g_ref = nn.net_backward(params, A, gY.astype(np.float64))
+    g = net.grad.cpu().numpy()
+    assert rel(g, g_ref) < 1e-4
+    for L in nn.LAYERS:  # per layer too (a wrong skip gradient would hide in the total)
+        o, nw, nb = net.offsets[L[0]]
+        assert rel(g[o:o + nw + nb], g_ref[o:o + nw + nb]) < 1e-3, L[0]
+
+
+def test_adam_matches_oracle(gpu):
+    import ctypes as C
+    import torch
+    from topoptlab_amd._lib import check, lib
+    rng = np.random.default_rng(9)
+    n = 1000
+    w, g = rng.standard_normal(n), rng.standard_normal(n)
+    m, v = np.zeros(n), np.zeros(n)
+    dw, dg = _t(w, gpu), _t(g, gpu)
+    dm, dv = torch.zeros(n, device=gpu), torch.zeros(n, device=gpu)
+    w16 = torch.empty(n, dtype=torch.float16, device=gpu)
+    s = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    p = lambda t: C.c_void_p(t.data_ptr())
+    for step in (1, 2, 3):
+        w, m, v = nn.adam(w, g, m, v, step, 1e-2)
+        check(lib().tl_nn_adam(n, p(dw), p(dg), p(dm), p(dv), p(w16), step, C.c_float(1e-2), s))
+    assert rel(dw.cpu().numpy(), w) < 1e-6
+    assert np.array_equal(w16.cpu().numpy(), dw.cpu().numpy().astype(np.float16))
+
+
+def test_training_reduces_loss_and_learned_guess_gains(gpu):
+    """Train on OC-run data of 32x16x16 cantilevers with seeded load
+    positions; on held-out load positions the learned initial guess must
+    reach the same tolerance in fewer PCG iterations than the zero guess
+    (the plain method), and the fallback (no network) is the plain solve."""
+    import torch
+    from topoptlab_amd import learned
+    from topoptlab_amd.elements import lk_linear_elast_3d
+    from topoptlab_amd.fem import StiffnessOperator
+    dims = (32, 16, 16)
+    say("learned: data")
+    X, T = learned.generate_pairs(*dims, runs=6, iters=8, seed=1, device=gpu)
+    net = learned.CoarseNet(gpu, seed=0)
+    say("learned: training on", len(X), "pairs")
+    hist = learned.train(net, X, T, epochs=25, lr=2e-3, seed=0)
+    say("learned: loss", hist[0], "->", hist[-1])
+    assert hist[-1] < 0.5 * hist[0]
+    held = []  # held-out OC runs (other seeded load positions)
+    learned.generate_pairs(*dims, runs=4, iters=8, seed=99, device=gpu, designs=held)
+    its0, its1 = [], []
+    for xPhys, f, fixed in held:
+        op = StiffnessOperator(*dims, KE=lk_linear_elast_3d(), fixed=fixed, device=gpu)
+        op.set_density(xPhys)
+        op.setup_mg(precision="fp32")
+        u0, i0 = op.solve(f, rtol=1e-8)
+        u1, i1 = learned.solve_with_learned_guess(net, op, xPhys, f, rtol=1e-8)
+        ub, ib = learned.solve_with_learned_guess(None, op, xPhys, f, rtol=1e-8)
+        assert i1["learned"] and not ib["learned"] and ib["iterations"] == i0["iterations"]
+        assert i1["relres_true"] < 1.1e-8 and i0["relres_true"] < 1.1e-8
+        assert rel(u1.cpu().numpy(), u0.cpu().numpy()) < 1e-6
+        its0.append(i0["iterations"])
+        its1.append(i1["iterations"])
+        op.close()
+    say("learned: PCG iterations zero guess", its0, "learned guess", its1)
+    assert np.mean(its1) < np.mean(its0)

@@ -69,6 +69,20 @@ _SIGS = {
     "tl_profile": (C.c_int, [C.c_void_p, C.c_int32]),
     "tl_profile_read": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                   C.POINTER(C.c_int64)]),
+    "tl_nn_conv3d": (C.c_int, [C.c_int32] * 7 + [C.c_void_p] * 5),
+    "tl_nn_conv3d_bwd_data": (C.c_int, [C.c_int32] * 6 + [C.c_void_p] * 3 + [C.c_int32,
+                                                                              C.c_void_p]),
+    "tl_nn_conv3d_bwd_weight": (C.c_int, [C.c_int32] * 6 + [C.c_void_p] * 5),
+    "tl_nn_relu_mask": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "tl_nn_upsample": (C.c_int, [C.c_int32] * 4 + [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
+    "tl_nn_upsample_t": (C.c_int, [C.c_int32] * 4 + [C.c_void_p, C.c_void_p, C.c_int32,
+                                                     C.c_void_p]),
+    "tl_nn_mse": (C.c_int, [C.c_int64] + [C.c_void_p] * 6),
+    "tl_nn_adam": (C.c_int, [C.c_int64] + [C.c_void_p] * 5 + [C.c_int32, C.c_float, C.c_void_p]),
+    "tl_nn_features": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
+    "tl_nn_inject": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
+    "tl_nn_prolong": (C.c_int, [C.c_void_p] * 3),
+    "tl_energy_scale": (C.c_int, [C.c_void_p] * 4 + [C.POINTER(C.c_double)]),
     "tl_mean_maxdiff": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double), C.c_void_p]),
 }
