@@ -195,9 +195,24 @@ def main():
         op.setup_mg(nlevels=args.levels, precision=vcycle, gamma=gamma, gamma_from=gamma_from)
         return op.solve(f, rtol=rtol, preconditioner="gmg", check_every=16)
 
-    def timed(vcycle, gamma, gamma_from, steps, warmup, profile=False):
+    net = None
+    wfile = ROOT / "topoptlab_amd" / "data" / "coarse_net.npz"
+    if wfile.exists() and not args.no_variants:
+        from topoptlab_amd import learned
+        net = learned.CoarseNet(dev)
+        net.load(np.load(wfile)["w"])
+
+    def step_learned(vcycle, gamma, gamma_from):
+        op.set_density(x)
+        op.setup_mg(nlevels=args.levels, precision=vcycle, gamma=gamma, gamma_from=gamma_from)
+        return learned.solve_with_learned_guess(net, op, x, f, rtol=1e-8, preconditioner="gmg",
+                                                check_every=16)
+
+    def timed(vcycle, gamma, gamma_from, steps, warmup, profile=False, use_net=False):
+        run = (lambda: step_learned(vcycle, gamma, gamma_from)) if use_net else \
+            (lambda: step(vcycle, gamma, gamma_from))
         for _ in range(warmup):
-            step(vcycle, gamma, gamma_from)
+            run()
         torch.cuda.synchronize()
         if profile:
             lib.tl_profile(op._h, 1)
@@ -205,7 +220,7 @@ def main():
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         for _ in range(steps):
-            u, info = step(vcycle, gamma, gamma_from)
+            u, info = run()
             infos.append(info)
         torch.cuda.synchronize()
         t = time.perf_counter() - t0
@@ -216,7 +231,8 @@ def main():
                 tot, cnt = C.c_double(), C.c_int64()
                 _lib.check(lib.tl_profile_read(op._h, kind, C.byref(tot), C.byref(cnt)))
                 prof[kind] = (tot.value / steps, cnt.value / steps)
-        return 1e3 * t / steps, u, infos[-1], prof
+        fallback = sum(1 for i in infos if not i.get("learned", True))
+        return 1e3 * t / steps, u, dict(infos[-1], fallbacks=fallback), prof
 
     # headline: the plain solve (FP64 V-cycle unless --vcycle says otherwise)
     ms, u, info, prof = timed(args.vcycle, args.gamma, args.gamma_from, args.steps, args.warmup,
@@ -254,10 +270,14 @@ def main():
         c_ref = float(torch.dot(f, u_ref).item())
         nu = torch.linalg.norm(u_ref).item()
         variants = {}
-        for name, vc, g, gf in (("fp64_vcycle", "fp64", 1, 1), ("fp32_vcycle", "fp32", 1, 1),
-                                ("bf16_mfma_vcycle", "bf16", 1, 1), ("fp16_mfma_vcycle", "fp16", 1, 1),
-                                ("fp16_mfma_wcycle_l2", "fp16", 2, 2)):
-            vms, vu, vinf, _ = timed(vc, g, gf, args.steps, 1)
+        forms = [("fp64_vcycle", "fp64", 1, 1, False), ("fp32_vcycle", "fp32", 1, 1, False),
+                 ("bf16_mfma_vcycle", "bf16", 1, 1, False), ("fp16_mfma_vcycle", "fp16", 1, 1, False),
+                 ("fp16_mfma_wcycle_l2", "fp16", 2, 2, False)]
+        if net is not None:
+            forms += [("fp64_vcycle_learned", "fp64", 1, 1, True),
+                      ("fp16_mfma_wcycle_l2_learned", "fp16", 2, 2, True)]
+        for name, vc, g, gf, use_net in forms:
+            vms, vu, vinf, _ = timed(vc, g, gf, args.steps, 1, use_net=use_net)
             vc_ = float(torch.dot(f, vu).item())
             variants[name] = {
                 "ms": round(vms, 3), "pcg_iterations": vinf["iterations"],
@@ -265,6 +285,9 @@ def main():
                 "compliance_rel_err": abs(vc_ - c_ref) / abs(c_ref),
                 "u_rel_err": torch.linalg.norm(vu - u_ref).item() / nu,
                 "speedup_vs_fp64_vcycle": None}
+            if use_net:
+                variants[name]["learned_fallbacks"] = f"{vinf['fallbacks']} of {args.steps}"
+                variants[name]["learned_alpha"] = vinf.get("alpha")
             log(f"variant {name}: {vms:.2f} ms, {vinf['iterations']} its, "
                 f"compliance err {variants[name]['compliance_rel_err']:.2e}, "
                 f"u err {variants[name]['u_rel_err']:.2e}")
