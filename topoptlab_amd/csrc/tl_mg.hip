@@ -378,59 +378,90 @@ __global__ void pad_identity_kernel(double* A, int64_t n, int64_t np) {
   if (i < np) A[i * np + i] = 1.0;
 }
 
-// step k, part 1: invert the diagonal GJB x GJB block in place
-__global__ __launch_bounds__(GJB* GJB) void gj_pivot_kernel(double* A, int64_t np, int k) {
-  __shared__ double a[GJB][GJB + 1];
-  const int i = threadIdx.x / GJB, j = threadIdx.x % GJB;
-  const int64_t kb = int64_t(k) * GJB;
-  a[i][j] = A[(kb + i) * np + kb + j];
-  __syncthreads();
+// Two-launch form of the blocked Gauss-Jordan step k (pivot block K =
+// [kb, kb+GJB)).  Launch 1 (one workgroup per 64 columns): every workgroup
+// inverts the pivot block in LDS (the same fixed arithmetic in each, so all
+// copies agree), writes the new row panel A[K][j] = P^-1 A[K][j] for its
+// columns j outside K and saves the old column panel of its 64 rows; the
+// first workgroup also saves P^-1.  Launch 2 (64x64 tiles): trailing update
+// A[i][j] -= C[i] R[j], column panel A[i][K] = -C[i] P^-1, pivot block P^-1.
+__device__ inline void gj_invert_pivot(double (*a)[GJB + 1]) {
+  // in-place Gauss-Jordan inverse of the GJB x GJB LDS block, BS threads
   for (int t = 0; t < GJB; ++t) {
-    const double att = a[t][t], ait = a[i][t], atj = a[t][j], aij = a[i][j];
-    double nv;
-    if (i == t && j == t) nv = 1.0 / att;
-    else if (i == t) nv = atj / att;
-    else if (j == t) nv = -ait / att;
-    else nv = aij - ait * atj / att;
+    double nv[GJB * GJB / BS];
+    const double att = a[t][t];
+#pragma unroll
+    for (int u = 0; u < GJB * GJB / BS; ++u) {
+      const int e = threadIdx.x + u * BS, i = e / GJB, j = e % GJB;
+      const double ait = a[i][t], atj = a[t][j], aij = a[i][j];
+      if (i == t && j == t) nv[u] = 1.0 / att;
+      else if (i == t) nv[u] = atj / att;
+      else if (j == t) nv[u] = -ait / att;
+      else nv[u] = aij - ait * atj / att;
+    }
     __syncthreads();
-    a[i][j] = nv;
+#pragma unroll
+    for (int u = 0; u < GJB * GJB / BS; ++u) {
+      const int e = threadIdx.x + u * BS;
+      a[e / GJB][e % GJB] = nv[u];
+    }
     __syncthreads();
   }
-  A[(kb + i) * np + kb + j] = a[i][j];
 }
 
-// part 2: row panel A[kb:kb+B, j] <- Pinv * A[kb:kb+B, j] for j outside the block
-__global__ __launch_bounds__(BS) void gj_rowpanel_kernel(double* A, int64_t np, int k) {
-  __shared__ double pv[GJB][GJB];
+// launch 1: workgroup b covers rows and columns [64b, 64b + 64); wave w
+// writes rows kb + 8w .. kb + 8w + 7 of the new row panel for its 64 columns
+constexpr int GJ_COLS = 64;
+__global__ __launch_bounds__(BS) void gj_panel_kernel(double* A, int64_t np, int k,
+                                                      double* __restrict__ colsave,
+                                                      double* __restrict__ pinv) {
+  static_assert(BS == 4 * GJ_COLS && GJB % 4 == 0, "four waves of 64 columns");
+  __shared__ double a[GJB][GJB + 1];
   const int64_t kb = int64_t(k) * GJB;
-  for (int t = threadIdx.x; t < GJB * GJB; t += BS) pv[t / GJB][t % GJB] = A[(kb + t / GJB) * np + kb + t % GJB];
-  __syncthreads();
-  const int64_t j = blockIdx.x * int64_t(BS) + threadIdx.x;
-  if (j >= np || (j >= kb && j < kb + GJB)) return;
+  const int64_t r0 = int64_t(blockIdx.x) * GJ_COLS;
+  for (int e = threadIdx.x; e < GJB * GJB; e += BS) a[e / GJB][e % GJB] = A[(kb + e / GJB) * np + kb + e % GJB];
+  // old column panel of the workgroup's 64 rows, 256-byte segments
+  for (int e = threadIdx.x; e < GJ_COLS * GJB; e += BS) {
+    const int64_t row = r0 + e / GJB;
+    if (row < np) colsave[row * GJB + e % GJB] = A[row * np + kb + e % GJB];
+  }
+  const int64_t j = r0 + threadIdx.x % GJ_COLS;
+  const bool active = j < np && (j < kb || j >= kb + GJB);
   double col[GJB];
 #pragma unroll
-  for (int t = 0; t < GJB; ++t) col[t] = A[(kb + t) * np + j];
-#pragma unroll 4
-  for (int r = 0; r < GJB; ++r) {
-    double s = 0.0;
+  for (int t = 0; t < GJB; ++t) col[t] = active ? A[(kb + t) * np + j] : 0.0;
+  __syncthreads();  // every old value read before any row-panel write below
+  gj_invert_pivot(a);
+  if (blockIdx.x == 0)
+    for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv[e] = a[e / GJB][e % GJB];
+  if (!active) return;
+  const int rb = (threadIdx.x / GJ_COLS) * (GJB / 4);
 #pragma unroll
-    for (int t = 0; t < GJB; ++t) s += pv[r][t] * col[t];
-    A[(kb + r) * np + j] = s;
+  for (int r = rb; r < rb + GJB / 4; ++r) {
+    double sum = 0.0;
+#pragma unroll
+    for (int t = 0; t < GJB; ++t) sum += a[r][t] * col[t];
+    A[(kb + r) * np + j] = sum;
   }
 }
 
-// part 3: A[i,j] -= A[i, kb:kb+B] * A[kb:kb+B, j] for i, j outside the block
-// (column panel still old, row panel already updated); 64x64 tiles
-__global__ __launch_bounds__(BS) void gj_update_kernel(double* A, int64_t np, int k) {
+__global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, int k,
+                                                        const double* __restrict__ colsave,
+                                                        const double* __restrict__ pinv) {
+  // X[t][j] = new row panel A[K][j] for j outside K, P^-1[t][j - kb] inside K;
+  // S = C X gives the trailing update (A -= S) and the column panel (A = -S)
   __shared__ double cp[64][GJB + 1];
-  __shared__ double rp[GJB][64 + 1];
+  __shared__ double xp[GJB][64 + 1];
   const int64_t kb = int64_t(k) * GJB;
   const int64_t i0 = int64_t(blockIdx.y) * 64, j0 = int64_t(blockIdx.x) * 64;
   for (int t = threadIdx.x; t < 64 * GJB; t += BS) {
     const int r = t / GJB, c = t % GJB;
-    cp[r][c] = (i0 + r < np) ? A[(i0 + r) * np + kb + c] : 0.0;
+    cp[r][c] = (i0 + r < np) ? colsave[(i0 + r) * GJB + c] : 0.0;
     const int r2 = t / 64, c2 = t % 64;
-    rp[r2][c2] = (j0 + c2 < np) ? A[(kb + r2) * np + j0 + c2] : 0.0;
+    const int64_t jj = j0 + c2;
+    double x = 0.0;
+    if (jj < np) x = (jj >= kb && jj < kb + GJB) ? pinv[r2 * GJB + (jj - kb)] : A[(kb + r2) * np + jj];
+    xp[r2][c2] = x;
   }
   __syncthreads();
   const int tj = threadIdx.x % 16, ti = threadIdx.x / 16;  // 16 x 16 threads, 4x4 each
@@ -439,7 +470,7 @@ __global__ __launch_bounds__(BS) void gj_update_kernel(double* A, int64_t np, in
   for (int t = 0; t < GJB; ++t) {
     double av[4], bv[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) { av[u] = cp[ti * 4 + u][t]; bv[u] = rp[t][tj * 4 + u]; }
+    for (int u = 0; u < 4; ++u) { av[u] = cp[ti * 4 + u][t]; bv[u] = xp[t][tj * 4 + u]; }
 #pragma unroll
     for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -448,33 +479,22 @@ __global__ __launch_bounds__(BS) void gj_update_kernel(double* A, int64_t np, in
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int64_t i = i0 + ti * 4 + u;
-    if (i >= np || (i >= kb && i < kb + GJB)) continue;
+    if (i >= np) continue;
+    const bool irow = i >= kb && i < kb + GJB;
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
       const int64_t j = j0 + tj * 4 + v;
-      if (j >= np || (j >= kb && j < kb + GJB)) continue;
-      A[i * np + j] -= acc[u][v];
+      if (j >= np) continue;
+      const bool jcol = j >= kb && j < kb + GJB;
+      if (irow) {
+        if (jcol) A[i * np + j] = pinv[(i - kb) * GJB + (j - kb)];
+        // rows in K outside the pivot columns: the new row panel, already written
+      } else if (jcol) {
+        A[i * np + j] = -acc[u][v];
+      } else {
+        A[i * np + j] -= acc[u][v];
+      }
     }
-  }
-}
-
-// part 4: column panel A[i, kb:kb+B] <- -A[i, kb:kb+B] * Pinv for i outside the block
-__global__ __launch_bounds__(BS) void gj_colpanel_kernel(double* A, int64_t np, int k) {
-  __shared__ double pv[GJB][GJB];
-  const int64_t kb = int64_t(k) * GJB;
-  for (int t = threadIdx.x; t < GJB * GJB; t += BS) pv[t / GJB][t % GJB] = A[(kb + t / GJB) * np + kb + t % GJB];
-  __syncthreads();
-  const int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x;
-  if (i >= np || (i >= kb && i < kb + GJB)) return;
-  double row[GJB];
-#pragma unroll
-  for (int t = 0; t < GJB; ++t) row[t] = A[i * np + kb + t];
-#pragma unroll 4
-  for (int c = 0; c < GJB; ++c) {
-    double s = 0.0;
-#pragma unroll
-    for (int t = 0; t < GJB; ++t) s += row[t] * pv[t][c];
-    A[i * np + kb + c] = -s;
   }
 }
 
@@ -827,13 +847,14 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
     }
     const int nbk = int(M.cnp / GJB);
     const unsigned nt = unsigned((M.cnp + 63) / 64);
+    TL_TRY(M.gjw.alloc(sizeof(double) * (size_t(M.cnp) * GJB + GJB * GJB)));
+    double* colsave = M.gjw.as<double>();
+    double* pinv = colsave + size_t(M.cnp) * GJB;
     for (int k = 0; k < nbk; ++k) {
-      hipLaunchKernelGGL(gj_pivot_kernel, dim3(1), dim3(GJB * GJB), 0, s, A, M.cnp, k);
-      hipLaunchKernelGGL(gj_rowpanel_kernel, dim3(unsigned((M.cnp + BS - 1) / BS)), dim3(BS), 0, s,
-                         A, M.cnp, k);
-      hipLaunchKernelGGL(gj_update_kernel, dim3(nt, nt), dim3(BS), 0, s, A, M.cnp, k);
-      hipLaunchKernelGGL(gj_colpanel_kernel, dim3(unsigned((M.cnp + BS - 1) / BS)), dim3(BS), 0, s,
-                         A, M.cnp, k);
+      hipLaunchKernelGGL(gj_panel_kernel, dim3(unsigned((M.cnp + GJ_COLS - 1) / GJ_COLS)), dim3(BS), 0, s, A,
+                         M.cnp, k, colsave, pinv);
+      hipLaunchKernelGGL(gj_update2_kernel, dim3(nt, nt), dim3(BS), 0, s, A, M.cnp, k, colsave,
+                         pinv);
     }
     TL_CHECK_LAUNCH();
   }
