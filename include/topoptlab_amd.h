@@ -103,6 +103,40 @@ typedef struct {
 int tl_solve(tl_problem* p, const double* f, double* u,
              const tl_solve_opts* opts, tl_solve_info* info);
 
+/* ---- Batched solves of many independent instances.  topoptlab solves one
+ * problem at a time and loops over load cases (solve_linsystem.py:341-352,
+ * topology_optimization.py:463-505); these calls take a whole batch. ---- */
+typedef struct {
+  int32_t nstreams;    /* concurrent HIP streams (<= 0: 4) */
+  int32_t chunk;       /* PCG iterations per host check (rounded up to even; <= 0: 8) */
+  int32_t use_graphs;  /* 1: each instance replays its chunk as a captured hipGraph */
+} tl_batch_opts;
+
+/* Solve K_ff u = f for nrhs[i] right-hand sides on each of n handles
+ * (density and, for GMG, tl_setup_mg already done).  f[i], u[i]: device
+ * float64 [nrhs[i]][ndof_i]; infos: sum(nrhs) entries, instance-major.
+ * Instances run concurrently on nstreams streams; the right-hand sides of
+ * one instance run one after another; every solve is the tl_solve PCG with
+ * the same arithmetic.  A handle may appear once.  TL_ENOCONV when any
+ * right-hand side did not converge (see its info.status). */
+int tl_batch_solve(int32_t n, tl_problem* const* handles, const double* const* f,
+                   double* const* u, const int32_t* nrhs, const tl_solve_opts* opts,
+                   const tl_batch_opts* bopts, tl_solve_info* infos);
+
+/* 2D operator workgroup shape: ty = 0 (a line of 256 consecutive nodes and
+ * a halo of 2*ny + 2 nodes) or a (256/ty) x ty node tile, ty in {16, 32,
+ * 64}; per precision TL_PREC_FP64 / TL_PREC_FP32.  Every shape gives the same
+ * arithmetic (same sums in the same order). */
+int tl_set_tile2d(tl_problem* p, int32_t prec, int32_t ty);
+/* Time every shape (reps launches each, FP64 and FP32 products) on this
+ * grid, keep the fastest in the handle and in a process-wide table keyed by
+ * (nx, ny, dofs per node); later calls for the same key reuse the table.
+ * tile_out[2] (may be NULL): chosen ty for FP64, FP32; us_out[8] (may be
+ * NULL): microseconds per launch, [fp64 ty=0,16,32,64, fp32 ...].  Grids
+ * with one shape (3D, or a non-pattern element matrix) return zeros. */
+int tl_autotune(tl_problem* p, int32_t reps, int32_t* tile_out, double* us_out);
+void tl_autotune_clear(void);
+
 /* Compliance and sensitivities summed over ncases load cases
  * (objectives.py:7-55, topology_optimization.py:463-505).
  * u: device [ncases][ndof]; xPhys, dc: device nel; obj: HOST scalar. */

@@ -36,6 +36,10 @@ class SolveOpts(C.Structure):
                 ("check_every", C.c_int32)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("nstreams", C.c_int32), ("chunk", C.c_int32), ("use_graphs", C.c_int32)]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("status", C.c_int32),
                 ("relres_recursive", C.c_double), ("relres_true", C.c_double)]
@@ -55,6 +59,12 @@ _SIGS = {
     "tl_setup_mg": (C.c_int, [C.c_void_p, C.POINTER(MgOpts)]),
     "tl_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(SolveOpts),
                            C.POINTER(SolveInfo)]),
+    "tl_batch_solve": (C.c_int, [C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                 C.POINTER(SolveOpts), C.POINTER(BatchOpts),
+                                 C.POINTER(SolveInfo)]),
+    "tl_set_tile2d": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "tl_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
+    "tl_autotune_clear": (None, []),
     "tl_compliance_sens": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                      C.POINTER(C.c_double), C.c_void_p]),
     "tl_filter_create": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_double,

@@ -17,6 +17,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace tl {
 
@@ -30,6 +33,11 @@ int DevBuf::alloc(size_t nbytes) {
   TL_HIP(hipMalloc(&ptr, nbytes));
   bytes = nbytes;
   return TL_OK;
+}
+
+void Level::release() {
+  mask.release(); st64.release(); sth64.release(); st32.release(); dinv64.release();
+  dinv32.release(); em.release(); vb.release(); vx.release(); vr.release(); vt.release();
 }
 
 void DevBuf::release() {
@@ -588,6 +596,90 @@ __global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
 
 static int launch_fold(tl_problem* P, unsigned nb, int fin, double* out, const int* done);
 
+// Rectangular-tile form of the 2D operator: a workgroup owns a TX x TY node
+// tile (TX = BS / TY, y fastest inside the tile) and stages its (TX+2) x
+// (TY+2) node halo once.  The node-line kernel above stages BS + 2*ny + 2
+// nodes for BS computed, a halo that grows with the grid height; the tile's
+// does not, but its rows are shorter.  tl_autotune picks the faster shape
+// per grid.
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
+__global__ __launch_bounds__(BS) void ku2t_kernel(KuArgs a) {
+  if (a.done && *a.done) return;
+  constexpr int TX = BS / TY;
+  constexpr int HY = TY + 2, HN = (TX + 2) * HY;
+  static_assert(TX * TY == BS, "tile must hold BS nodes");
+  __shared__ T lds[HN * NDPN];
+  const int ntx = (a.nx + TX - 1) / TX;
+  const int x0 = int(blockIdx.x % unsigned(ntx)) * TX, y0 = int(blockIdx.x / unsigned(ntx)) * TY;
+  const int lx = threadIdx.x / TY, ly = threadIdx.x % TY;
+  const int ix = x0 + lx, iy = y0 + ly;
+  const bool active = ix < a.nx && iy < a.ny;
+  for (int t = threadIdx.x; t < HN * NDPN; t += BS) {
+    const int h = t / NDPN, c = t % NDPN;
+    const int hx = x0 - 1 + h / HY, hy = y0 - 1 + h % HY;
+    T v = T(0);
+    if (hx >= 0 && hx < a.nx && hy >= 0 && hy < a.ny)
+      v = stage_val<T>(a, (int64_t(hx) * a.ny + hy) * NDPN + c, FUSEP);
+    lds[t] = v;
+  }
+  T U[3][9][NDPN];
+  T S[3][4];
+  {
+    // padded scale: (nelx+2) rows of (nely+2); inactive threads read node 0's
+    const T* sp = static_cast<const T*>(a.sp);
+    const int cx = active ? ix : 0, cy = active ? iy : 0;
+#pragma unroll
+    for (int ox = -1; ox <= 0; ++ox)
+#pragma unroll
+      for (int oy = -1; oy <= 0; ++oy)
+        S[1][(ox + 1) * 2 + (oy + 1)] = sp[int64_t(cx + ox + 1) * (a.nely + 2) + cy + oy + 1];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy)
+#pragma unroll
+      for (int c = 0; c < NDPN; ++c)
+        U[1][(dx + 1) * 3 + (dy + 1)][c] = lds[((lx + 1 + dx) * HY + ly + 1 + dy) * NDPN + c];
+  T acc[NDPN];
+  node_sum<NDPN, T, 0, 1, 2, false, true>(a, U, S, 0, 1, acc);
+  double dot = 0.0;
+  if (active) {
+    const int64_t j = int64_t(ix) * a.ny + iy;
+    T uc[NDPN];
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d) uc[d] = U[1][4][d];
+    if constexpr (FUSEP) {
+#pragma unroll
+      for (int d = 0; d < NDPN; ++d) static_cast<T*>(a.pout)[j * NDPN + d] = uc[d];
+    }
+    ku_epilogue<NDPN, T, MODE, TB, TO>(a, j, uc, acc, dot);
+  }
+  if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
+    __shared__ double sh[BS / 64 + 2];
+    double v[1] = {dot};
+    block_sum<1>(v, sh);
+    store_partials<1>(v, a.partials);
+  }
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
+static int launch_ku2t(const tl_problem* P, const KuArgs& a) {
+  constexpr int TX = BS / TY;
+  const int64_t nb = int64_t((a.nx + TX - 1) / TX) * ((a.ny + TY - 1) / TY);
+  if (nb > kMaxRedBlocks && (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)) {
+    set_error("fine operator grid exceeds the reduction slot count");
+    return TL_EINVAL;
+  }
+  hipLaunchKernelGGL((ku2t_kernel<NDPN, T, MODE, TB, TO, FUSEP, TY>), dim3(unsigned(nb)), dim3(BS),
+                     0, P->stream, a);
+  TL_CHECK_LAUNCH();
+  if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)
+    return launch_fold(const_cast<tl_problem*>(P), unsigned(nb), a.fin, a.dot_out, a.done);
+  return TL_OK;
+}
+
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
 static int launch_ku_impl(const tl_problem* P, KuArgs a) {
 
@@ -616,6 +708,14 @@ static int launch_ku_impl(const tl_problem* P, KuArgs a) {
     }
   } else {
     a.zchunk = 1;
+    if constexpr (NDPN <= 2 && FAST) {
+      switch (P->op.tile2d[sizeof(T) == 8 ? 0 : 1]) {
+        case 16: return launch_ku2t<NDPN, T, MODE, TB, TO, FUSEP, 16>(P, a);
+        case 32: return launch_ku2t<NDPN, T, MODE, TB, TO, FUSEP, 32>(P, a);
+        case 64: return launch_ku2t<NDPN, T, MODE, TB, TO, FUSEP, 64>(P, a);
+        default: break;
+      }
+    }
     if (nbx > unsigned(kMaxRedBlocks) && (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)) {
       set_error("fine operator grid exceeds the reduction slot count");
       return TL_EINVAL;
@@ -965,13 +1065,10 @@ tl_problem::~tl_problem() {
   op.KE64.release(); op.KE32.release(); op.KE16.release();
   op.sp64.release(); op.sp32.release();
   op.dinv64.release(); op.dinv32.release(); op.mask.release();
-  for (auto& l : mg.lv) {
-    l.mask.release(); l.st64.release(); l.st32.release(); l.dinv64.release();
-    l.dinv32.release(); l.em.release(); l.vb.release(); l.vx.release(); l.vr.release();
-    l.vt.release();
-  }
-  mg.G1.release(); mg.W.release(); mg.cinv.release();
-  r.release(); z.release(); p.release(); p2.release(); q.release(); t.release();
+  if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+  for (auto& l : mg.lv) l.release();
+  mg.G1.release(); mg.W.release(); mg.cinv.release(); mg.gjw.release();
+  r.release(); z.release(); p.release(); p2.release(); q.release(); t.release(); xb.release();
   partials.release(); state.release();
 }
 
@@ -1167,6 +1264,123 @@ int tl_diag_K(tl_problem* P, double* d) {
   return fine_diag_to(P, d, nullptr, nullptr);
 }
 
+}  // extern "C"
+
+namespace tl {
+namespace {
+
+// Buffers and launch shapes of one PCG solve on a handle.
+struct PcgCtx {
+  int64_t n = 0;
+  double *r = nullptr, *z = nullptr, *q = nullptr, *t = nullptr;
+  double* pp[2] = {nullptr, nullptr};  // search directions, ping-pong
+  PcgState* st = nullptr;
+  const int* done = nullptr;
+  double* partials = nullptr;
+  const uint8_t* mask = nullptr;
+  unsigned nb = 0, rb = 0;
+};
+
+int pcg_context(tl_problem* P, PcgCtx& c) {
+  const int64_t n = P->ndof;
+  for (DevBuf* b : {&P->r, &P->z, &P->p, &P->p2, &P->q, &P->t}) TL_TRY(b->alloc(sizeof(double) * n));
+  c.n = n;
+  c.r = P->r.as<double>(); c.z = P->z.as<double>(); c.q = P->q.as<double>(); c.t = P->t.as<double>();
+  c.pp[0] = P->p.as<double>(); c.pp[1] = P->p2.as<double>();
+  c.st = P->state.as<PcgState>();
+  c.done = &c.st->done;
+  c.partials = P->partials.as<double>();
+  c.mask = P->op.mask.as<uint8_t>();
+  c.nb = grid_for(n); c.rb = red_blocks(n);
+  return TL_OK;
+}
+
+// t = masked f, bb, state, x = masked x0 or 0, r = t - K x, rr0, p = 0.
+int pcg_init(tl_problem* P, const PcgCtx& c, const double* f, const double* x0, double* x,
+             const tl_solve_opts* o) {
+  hipStream_t s = P->stream;
+  const int64_t n = c.n;
+  hipLaunchKernelGGL(masked_copy_kernel, dim3(c.nb), dim3(BS), 0, s, f, c.t, c.mask, P->ndpn, n);
+  TL_CHECK_LAUNCH();
+  hipLaunchKernelGGL(dot_kernel, dim3(c.rb), dim3(BS), 0, s, c.t, c.t, n, c.partials);
+  TL_CHECK_LAUNCH();
+  TL_TRY(launch_fold(P, c.rb, FIN_BB, nullptr, nullptr));
+  hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(1), 0, s, c.st, o->rtol, o->atol, o->maxiter);
+  TL_CHECK_LAUNCH();
+  if (x0) {
+    hipLaunchKernelGGL(masked_copy_kernel, dim3(c.nb), dim3(BS), 0, s, x0, x, c.mask, P->ndpn, n);
+    TL_CHECK_LAUNCH();
+    TL_TRY(fine_apply_ex(P, x, c.q, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0, nullptr,
+                         nullptr, nullptr, nullptr, FIN_STORE, nullptr, true));
+  } else {
+    TL_HIP(hipMemsetAsync(x, 0, sizeof(double) * n, s));
+    TL_HIP(hipMemsetAsync(c.q, 0, sizeof(double) * n, s));
+  }
+  hipLaunchKernelGGL(resid_kernel, dim3(c.rb), dim3(BS), 0, s, c.t, c.q, c.mask, P->ndpn, c.r, n,
+                     c.partials);
+  TL_CHECK_LAUNCH();
+  TL_TRY(launch_fold(P, c.rb, FIN_RR0, nullptr, nullptr));
+  TL_HIP(hipMemsetAsync(c.pp[0], 0, sizeof(double) * n, s));
+  TL_HIP(hipMemsetAsync(c.pp[1], 0, sizeof(double) * n, s));
+  return TL_OK;
+}
+
+// One PCG iteration (k counts from 0 after pcg_init).  Every kernel returns
+// at once when the device state says done, so iterations launched past
+// convergence change nothing.  No allocation, memset or host transfer: the
+// sequence can be captured into a graph.
+int pcg_iteration(tl_problem* P, const PcgCtx& c, double* x, int precond, int64_t k) {
+  hipStream_t s = P->stream;
+  // z = M r ; rho = r.z ; beta
+  if (precond == TL_PRECOND_GMG) {
+    TL_TRY(mg_vcycle(P, c.r, c.z, c.done));
+  } else {
+    hipLaunchKernelGGL(jacobi_prec_kernel, dim3(c.rb), dim3(BS), 0, s, c.r,
+                       precond == TL_PRECOND_JACOBI ? P->op.dinv64.as<double>() : nullptr, c.z,
+                       c.n, c.partials, c.done);
+    TL_CHECK_LAUNCH();
+    TL_TRY(launch_fold(P, c.rb, FIN_RHO, nullptr, c.done));
+  }
+  // p = z + beta p ; q = K p ; pq ; alpha.  The fused product reads p_k
+  // (halo included) while it writes p_{k+1}: two buffers.
+  double* pin = c.pp[k & 1];
+  double* pout = c.pp[(k + 1) & 1];
+  TL_TRY(fine_apply_ex(P, c.z, c.q, TL_PREC_FP64, KU_STORE_DOT, nullptr, false, nullptr, 0.0, pin,
+                       pout, &c.st->beta, nullptr, FIN_PQ, c.done, false));
+  // x += alpha p ; r -= alpha q ; rr ; done
+  hipLaunchKernelGGL(update_kernel, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n, c.partials,
+                     c.st, c.done);
+  TL_CHECK_LAUNCH();
+  return launch_fold(P, c.rb, FIN_RR, nullptr, c.done);
+}
+
+// True residual |t - K x|^2 into *tr2_dev (a device slot), state unchanged.
+int pcg_true_residual(tl_problem* P, const PcgCtx& c, const double* x, double* tr2_dev) {
+  TL_TRY(fine_apply_ex(P, x, c.q, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0, nullptr,
+                       nullptr, nullptr, nullptr, FIN_STORE, nullptr, true));
+  hipLaunchKernelGGL(resid_kernel, dim3(c.rb), dim3(BS), 0, P->stream, c.t, c.q, c.mask, P->ndpn,
+                     static_cast<double*>(nullptr), c.n, c.partials);
+  TL_CHECK_LAUNCH();
+  return launch_fold(P, c.rb, FIN_STORE, tr2_dev, nullptr);
+}
+
+bool pcg_info(const PcgState& hs, double tr2, tl_solve_info* info) {
+  const bool conv = hs.rr < hs.tol2;
+  if (info) {
+    const double bn = std::sqrt(hs.bb);
+    info->iterations = hs.it;
+    info->status = conv ? 0 : TL_ENOCONV;
+    info->relres_recursive = bn > 0 ? std::sqrt(hs.rr) / bn : 0.0;
+    info->relres_true = bn > 0 ? std::sqrt(tr2) / bn : 0.0;
+  }
+  return conv;
+}
+
+}  // namespace
+}  // namespace tl
+
+extern "C" {
+
 int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
              tl_solve_info* info) {
   TL_ARG(P && f && u && o, "null argument");
@@ -1174,90 +1388,25 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
   TL_ARG(o->precond >= TL_PRECOND_NONE && o->precond <= TL_PRECOND_GMG, "unknown preconditioner");
   TL_ARG(o->precond != TL_PRECOND_GMG || P->mg.ready, "tl_setup_mg must be called first");
   TL_ARG(o->rtol >= 0.0 && o->atol >= 0.0, "tolerances must be non-negative");
-  const int64_t n = P->ndof;
-  TL_TRY(P->r.alloc(sizeof(double) * n));
-  TL_TRY(P->z.alloc(sizeof(double) * n));
-  TL_TRY(P->p.alloc(sizeof(double) * n));
-  TL_TRY(P->p2.alloc(sizeof(double) * n));
-  TL_TRY(P->q.alloc(sizeof(double) * n));
-  TL_TRY(P->t.alloc(sizeof(double) * n));
-  double* r = P->r.as<double>();
-  double* z = P->z.as<double>();
-  double* p = P->p.as<double>();
-  double* q = P->q.as<double>();
-  double* t = P->t.as<double>();
-  PcgState* st = P->state.as<PcgState>();
-  const int* done = &st->done;
-  double* partials = P->partials.as<double>();
-  const uint8_t* mask = P->op.mask.as<uint8_t>();
-  const unsigned nb = grid_for(n), rb = red_blocks(n);
+  PcgCtx c;
+  TL_TRY(pcg_context(P, c));
   hipStream_t s = P->stream;
-  // b (masked) -> t ; bb
-  hipLaunchKernelGGL(masked_copy_kernel, dim3(nb), dim3(BS), 0, s, f, t, mask, P->ndpn, n);
-  TL_CHECK_LAUNCH();
-  hipLaunchKernelGGL(dot_kernel, dim3(rb), dim3(BS), 0, s, t, t, n, partials);
-  TL_CHECK_LAUNCH();
-  TL_TRY(launch_fold(P, rb, FIN_BB, nullptr, nullptr));
-  hipLaunchKernelGGL(init_state_kernel, dim3(1), dim3(1), 0, s, st, o->rtol, o->atol, o->maxiter);
-  TL_CHECK_LAUNCH();
-  // x0: masked copy of u, or zero
-  if (o->x0_given) {
-    hipLaunchKernelGGL(masked_copy_kernel, dim3(nb), dim3(BS), 0, s, u, u, mask, P->ndpn, n);
-    TL_CHECK_LAUNCH();
-    TL_TRY(fine_apply_ex(P, u, q, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0, nullptr,
-                         nullptr, nullptr, nullptr, FIN_STORE, nullptr, true));
-    hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn, r, n,
-                       partials);
-  } else {
-    TL_HIP(hipMemsetAsync(u, 0, sizeof(double) * n, s));
-    TL_HIP(hipMemsetAsync(q, 0, sizeof(double) * n, s));
-    hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn, r, n,
-                       partials);
-  }
-  TL_CHECK_LAUNCH();
-  TL_TRY(launch_fold(P, rb, FIN_RR0, nullptr, nullptr));
-  // search directions ping-pong between two buffers: the fused product
-  // reads p_k (halo included) while it writes p_{k+1}
-  double* pp[2] = {p, P->p2.as<double>()};
-  TL_HIP(hipMemsetAsync(pp[0], 0, sizeof(double) * n, s));
-  TL_HIP(hipMemsetAsync(pp[1], 0, sizeof(double) * n, s));
+  TL_TRY(pcg_init(P, c, f, o->x0_given ? u : nullptr, u, o));
   // Iterations run in chunks between host checks of the device state; after
   // the first chunk the chunk length follows the observed residual decay so
   // that few iterations are launched past convergence (they exit at once but
   // still cost a launch each).
   int chunk = std::max(1, o->check_every);
   PcgState hs{};
-  TL_HIP(hipMemcpyAsync(&hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  TL_HIP(hipMemcpyAsync(&hs, c.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
   TL_HIP(hipStreamSynchronize(s));
   double rr_prev = hs.rr;
   int it_prev = 0;
   int64_t kk = 0;
   for (int launched = 0; !hs.done;) {
-    for (int k = 0; k < chunk; ++k) {
-      // z = M r ; rho = r.z ; beta
-      if (o->precond == TL_PRECOND_GMG) {
-        TL_TRY(mg_vcycle(P, r, z, done));
-      } else {
-        hipLaunchKernelGGL(jacobi_prec_kernel, dim3(rb), dim3(BS), 0, s, r,
-                           o->precond == TL_PRECOND_JACOBI ? P->op.dinv64.as<double>() : nullptr,
-                           z, n, partials, done);
-        TL_CHECK_LAUNCH();
-        TL_TRY(launch_fold(P, rb, FIN_RHO, nullptr, done));
-      }
-      // p = z + beta p ; q = K p ; pq ; alpha
-      double* pin = pp[kk & 1];
-      double* pout = pp[(kk + 1) & 1];
-      ++kk;
-      TL_TRY(fine_apply_ex(P, z, q, TL_PREC_FP64, KU_STORE_DOT, nullptr, false, nullptr, 0.0, pin,
-                           pout, &st->beta, nullptr, FIN_PQ, done, false));
-      // x += alpha p ; r -= alpha q ; rr ; done
-      hipLaunchKernelGGL(update_kernel, dim3(rb), dim3(BS), 0, s, u, r, pout, q, n, partials,
-                         st, done);
-      TL_CHECK_LAUNCH();
-      TL_TRY(launch_fold(P, rb, FIN_RR, nullptr, done));
-    }
+    for (int k = 0; k < chunk; ++k) TL_TRY(pcg_iteration(P, c, u, o->precond, kk++));
     launched += chunk;
-    TL_HIP(hipMemcpyAsync(&hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+    TL_HIP(hipMemcpyAsync(&hs, c.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
     TL_HIP(hipStreamSynchronize(s));
     if (hs.done) break;
     if (launched > o->maxiter + chunk) break;  // defensive: done must have been set
@@ -1271,28 +1420,305 @@ int tl_solve(tl_problem* P, const double* f, double* u, const tl_solve_opts* o,
     rr_prev = hs.rr;
     it_prev = hs.it;
   }
-  // true residual in fp64
-  TL_TRY(fine_apply_ex(P, u, q, TL_PREC_FP64, KU_STORE, nullptr, false, nullptr, 0.0, nullptr,
-                       nullptr, nullptr, nullptr, FIN_STORE, nullptr, true));
-  double* scal = P->partials.as<double>() + 3 * kMaxRedBlocks;  // scratch slot for the total
-  hipLaunchKernelGGL(resid_kernel, dim3(rb), dim3(BS), 0, s, t, q, mask, P->ndpn,
-                     static_cast<double*>(nullptr), n, partials);
-  TL_CHECK_LAUNCH();
-  TL_TRY(launch_fold(P, rb, FIN_STORE, scal, nullptr));
+  double* scal = c.partials + 3 * kMaxRedBlocks;  // scratch slot for the total
+  TL_TRY(pcg_true_residual(P, c, u, scal));
   double tr2 = 0.0;
   TL_HIP(hipMemcpyAsync(&tr2, scal, sizeof(double), hipMemcpyDeviceToHost, s));
-  TL_HIP(hipMemcpyAsync(&hs, st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
+  TL_HIP(hipMemcpyAsync(&hs, c.st, sizeof(PcgState), hipMemcpyDeviceToHost, s));
   TL_HIP(hipStreamSynchronize(s));
-  const bool conv = hs.rr < hs.tol2;
-  if (info) {
-    const double bn = std::sqrt(hs.bb);
-    info->iterations = hs.it;
-    info->status = conv ? 0 : TL_ENOCONV;
-    info->relres_recursive = bn > 0 ? std::sqrt(hs.rr) / bn : 0.0;
-    info->relres_true = bn > 0 ? std::sqrt(tr2) / bn : 0.0;
-  }
-  if (!conv) {
+  if (!pcg_info(hs, tr2, info)) {
     set_error("pcg iteration did not converge");
+    return TL_ENOCONV;
+  }
+  return TL_OK;
+}
+
+
+// ----------------------------------------------------------------------------
+// autotuner of the 2D operator's workgroup shape
+// ----------------------------------------------------------------------------
+namespace {
+struct TuneKey {
+  int nx, ny, ndpn;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(nx, ny, ndpn) < std::tie(o.nx, o.ny, o.ndpn);
+  }
+};
+struct TuneVal {
+  int tile[2];
+  double us[2][4];
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+constexpr int kTiles[4] = {0, 16, 32, 64};
+}  // namespace
+
+int tl_set_tile2d(tl_problem* P, int32_t prec, int32_t ty) {
+  TL_ARG(P, "null argument");
+  TL_ARG(P->g.dim == 2, "the tile shape applies to 2D grids");
+  TL_ARG(prec == TL_PREC_FP64 || prec == TL_PREC_FP32, "tile shapes exist for FP64 and FP32");
+  TL_ARG(ty == 0 || ty == 16 || ty == 32 || ty == 64, "tile height must be 0, 16, 32 or 64");
+  TL_ARG(ty == 0 || P->op.fast_ke, "tiles need the pattern-matched element matrix");
+  P->op.tile2d[prec == TL_PREC_FP64 ? 0 : 1] = ty;
+  return TL_OK;
+}
+
+int tl_autotune(tl_problem* P, int32_t reps, int32_t* tile_out, double* us_out) {
+  TL_ARG(P, "null argument");
+  TL_ARG(P->op.have_density, "tl_set_density must be called first");
+  TL_ARG(reps >= 1 && reps <= 1000, "reps must be in [1, 1000]");
+  if (P->g.dim != 2 || !P->op.fast_ke) {  // one shape only: nothing to choose
+    if (tile_out) tile_out[0] = tile_out[1] = 0;
+    if (us_out) for (int k = 0; k < 8; ++k) us_out[k] = 0.0;
+    return TL_OK;
+  }
+  const TuneKey key{P->g.nx, P->g.ny, P->ndpn};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      P->op.tile2d[0] = it->second.tile[0];
+      P->op.tile2d[1] = it->second.tile[1];
+      if (tile_out) { tile_out[0] = it->second.tile[0]; tile_out[1] = it->second.tile[1]; }
+      if (us_out) for (int k = 0; k < 8; ++k) us_out[k] = it->second.us[k / 4][k % 4];
+      return TL_OK;
+    }
+  }
+  const int64_t n = P->ndof;
+  TL_TRY(P->t.alloc(sizeof(double) * n));
+  TL_TRY(P->q.alloc(sizeof(double) * n));
+  hipStream_t s = P->stream;
+  TL_HIP(hipMemsetAsync(P->t.ptr, 0, sizeof(double) * n, s));
+  hipEvent_t e0, e1;
+  TL_HIP(hipEventCreate(&e0));
+  TL_HIP(hipEventCreate(&e1));
+  TuneVal tv{};
+  const bool prof = P->prof_on;
+  P->prof_on = false;
+  int rc = TL_OK;
+  for (int pi = 0; pi < 2 && rc == TL_OK; ++pi) {
+    const int prec = pi == 0 ? TL_PREC_FP64 : TL_PREC_FP32;
+    double best = 1e300;
+    for (int ci = 0; ci < 4 && rc == TL_OK; ++ci) {
+      P->op.tile2d[pi] = kTiles[ci];
+      auto run = [&]() {
+        return fine_apply_ex(P, P->t.ptr, P->q.ptr, prec, KU_STORE, nullptr, false, nullptr, 0.0,
+                             nullptr, nullptr, nullptr, nullptr, FIN_STORE, nullptr, false);
+      };
+      if ((rc = run()) != TL_OK) break;  // warm-up
+      if (hipEventRecord(e0, s) != hipSuccess) { rc = TL_EHIP; break; }
+      for (int k = 0; k < reps && rc == TL_OK; ++k) rc = run();
+      if (rc != TL_OK) break;
+      float ms = 0.f;
+      if (hipEventRecord(e1, s) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+        set_error("autotune timing failed");
+        rc = TL_EHIP;
+        break;
+      }
+      tv.us[pi][ci] = 1e3 * ms / reps;
+      if (tv.us[pi][ci] < best) { best = tv.us[pi][ci]; tv.tile[pi] = kTiles[ci]; }
+    }
+    P->op.tile2d[pi] = tv.tile[pi];
+  }
+  P->prof_on = prof;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc != TL_OK) {
+    P->op.tile2d[0] = P->op.tile2d[1] = 0;
+    return rc;
+  }
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune[key] = tv;
+  }
+  if (tile_out) { tile_out[0] = tv.tile[0]; tile_out[1] = tv.tile[1]; }
+  if (us_out) for (int k = 0; k < 8; ++k) us_out[k] = tv.us[k / 4][k % 4];
+  return TL_OK;
+}
+
+void tl_autotune_clear(void) {
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune.clear();
+}
+
+// ----------------------------------------------------------------------------
+// batched solves
+// ----------------------------------------------------------------------------
+namespace {
+// Capture `chunk` PCG iterations of handle P (iterate in P->xb) on stream s
+// into an executable graph, or reuse the one captured with the same key.
+int chunk_graph(tl_problem* P, const PcgCtx& c, int precond, int chunk) {
+  const int64_t key[5] = {int64_t(P->gen), P->op.tile2d[0], P->op.tile2d[1], chunk, precond};
+  if (P->chunk_exec && std::equal(key, key + 5, P->chunk_key)) return TL_OK;
+  if (P->chunk_exec) {
+    (void)hipGraphExecDestroy(P->chunk_exec);
+    P->chunk_exec = nullptr;
+  }
+  hipStream_t s = P->stream;
+  TL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  int rc = TL_OK;
+  for (int k = 0; k < chunk && rc == TL_OK; ++k)
+    rc = pcg_iteration(P, c, P->xb.as<double>(), precond, k);
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(s, &g);
+  if (rc != TL_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  TL_HIP(ec);
+  const hipError_t ei = hipGraphInstantiate(&P->chunk_exec, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  TL_HIP(ei);
+  std::copy(key, key + 5, P->chunk_key);
+  return TL_OK;
+}
+
+struct BatchSlot {
+  PcgState hs;
+  double tr2;
+  double pad[7];
+};
+}  // namespace
+
+int tl_batch_solve(int32_t n, tl_problem* const* hs, const double* const* f, double* const* u,
+                   const int32_t* nrhs, const tl_solve_opts* o, const tl_batch_opts* bo,
+                   tl_solve_info* infos) {
+  TL_ARG(n >= 1 && hs && f && u && nrhs && o, "null argument or empty batch");
+  TL_ARG(o->precond >= TL_PRECOND_NONE && o->precond <= TL_PRECOND_GMG, "unknown preconditioner");
+  TL_ARG(o->rtol >= 0.0 && o->atol >= 0.0, "tolerances must be non-negative");
+  const int nstreams = std::max(1, std::min<int>(n, bo && bo->nstreams > 0 ? bo->nstreams : 4));
+  int chunk = bo && bo->chunk > 0 ? bo->chunk : 8;
+  chunk += chunk & 1;  // even: each chunk starts on the same search-direction buffer
+  TL_ARG(chunk <= 256, "chunk must be <= 256");
+  const bool graphs = !bo || bo->use_graphs;
+  for (int i = 0; i < n; ++i) {
+    TL_ARG(hs[i] && f[i] && u[i], "null handle or vector");
+    TL_ARG(nrhs[i] >= 1, "every instance needs at least one right-hand side");
+    TL_ARG(hs[i]->op.have_density, "tl_set_density must be called first");
+    TL_ARG(o->precond != TL_PRECOND_GMG || hs[i]->mg.ready, "tl_setup_mg must be called first");
+    for (int j = 0; j < i; ++j) TL_ARG(hs[i] != hs[j], "a handle appears twice in the batch");
+  }
+  std::vector<hipStream_t> streams(nstreams, nullptr);
+  std::vector<hipEvent_t> ev(n, nullptr);
+  std::vector<hipStream_t> saved(n);
+  std::vector<bool> saved_prof(n);
+  std::vector<PcgCtx> ctx(n);
+  BatchSlot* slots = nullptr;
+  int rc = TL_OK;
+  auto fail = [&](int code) { if (rc == TL_OK) rc = code; };
+  for (int i = 0; i < n; ++i) {
+    saved[i] = hs[i]->stream;
+    saved_prof[i] = hs[i]->prof_on;
+  }
+  if (hipHostMalloc(reinterpret_cast<void**>(&slots), sizeof(BatchSlot) * n, hipHostMallocDefault) !=
+      hipSuccess) {
+    set_error("hipHostMalloc of the batch state slots failed");
+    return TL_EHIP;
+  }
+  for (int k = 0; k < nstreams && rc == TL_OK; ++k)
+    if (hipStreamCreateWithFlags(&streams[k], hipStreamNonBlocking) != hipSuccess) fail(TL_EHIP);
+  for (int i = 0; i < n && rc == TL_OK; ++i)
+    if (hipEventCreateWithFlags(&ev[i], hipEventDisableTiming) != hipSuccess) fail(TL_EHIP);
+  if (rc != TL_OK) set_error("stream or event creation failed");
+  // per instance: r index, iterations launched, phase (0 iterate, 1 finish, 2 done)
+  std::vector<int> rhs(n, 0), launched(n, 0), phase(n, 0);
+  std::vector<int64_t> info_at(n, 0);
+  for (int i = 1; i < n; ++i) info_at[i] = info_at[i - 1] + nrhs[i - 1];
+  auto launch_chunk = [&](int i) -> int {
+    tl_problem* P = hs[i];
+    if (graphs) {
+      TL_TRY(chunk_graph(P, ctx[i], o->precond, chunk));
+      TL_HIP(hipGraphLaunch(P->chunk_exec, P->stream));
+    } else {
+      for (int k = 0; k < chunk; ++k) TL_TRY(pcg_iteration(P, ctx[i], P->xb.as<double>(), o->precond, k));
+    }
+    launched[i] += chunk;
+    TL_HIP(hipMemcpyAsync(&slots[i].hs, ctx[i].st, sizeof(PcgState), hipMemcpyDeviceToHost, P->stream));
+    TL_HIP(hipEventRecord(ev[i], P->stream));
+    return TL_OK;
+  };
+  auto start_rhs = [&](int i) -> int {
+    tl_problem* P = hs[i];
+    const int64_t nd = P->ndof;
+    const double* x0 = o->x0_given ? u[i] + int64_t(rhs[i]) * nd : nullptr;
+    TL_TRY(pcg_init(P, ctx[i], f[i] + int64_t(rhs[i]) * nd, x0, P->xb.as<double>(), o));
+    launched[i] = 0;
+    phase[i] = 0;
+    return launch_chunk(i);
+  };
+  auto finish_rhs = [&](int i) -> int {
+    tl_problem* P = hs[i];
+    double* scal = ctx[i].partials + 3 * kMaxRedBlocks;
+    TL_TRY(pcg_true_residual(P, ctx[i], P->xb.as<double>(), scal));
+    TL_HIP(hipMemcpyAsync(u[i] + int64_t(rhs[i]) * P->ndof, P->xb.ptr, sizeof(double) * P->ndof,
+                          hipMemcpyDeviceToDevice, P->stream));
+    TL_HIP(hipMemcpyAsync(&slots[i].tr2, scal, sizeof(double), hipMemcpyDeviceToHost, P->stream));
+    TL_HIP(hipMemcpyAsync(&slots[i].hs, ctx[i].st, sizeof(PcgState), hipMemcpyDeviceToHost,
+                          P->stream));
+    TL_HIP(hipEventRecord(ev[i], P->stream));
+    phase[i] = 1;
+    return TL_OK;
+  };
+  bool all_conv = true;
+  // batch streams start after the work already queued on each handle's stream
+  for (int i = 0; i < n && rc == TL_OK; ++i)
+    if (hipEventRecord(ev[i], saved[i]) != hipSuccess ||
+        hipStreamWaitEvent(streams[i % nstreams], ev[i], 0) != hipSuccess) {
+      set_error("ordering the batch streams after the handles' streams failed");
+      fail(TL_EHIP);
+    }
+  if (rc == TL_OK) {
+    for (int i = 0; i < n && rc == TL_OK; ++i) {
+      tl_problem* P = hs[i];
+      P->stream = streams[i % nstreams];
+      P->prof_on = false;  // no event records inside captured chunks
+      int e = pcg_context(P, ctx[i]);
+      if (e == TL_OK) e = P->xb.alloc(sizeof(double) * P->ndof);
+      if (e == TL_OK) e = start_rhs(i);
+      fail(e);
+    }
+    int remaining = n;
+    while (rc == TL_OK && remaining > 0) {
+      for (int i = 0; i < n && rc == TL_OK; ++i) {
+        if (phase[i] == 2) continue;
+        const hipError_t q = hipEventQuery(ev[i]);
+        if (q == hipErrorNotReady) continue;
+        if (q != hipSuccess) {
+          set_error(std::string("batch event: ") + hipGetErrorString(q));
+          fail(TL_EHIP);
+          break;
+        }
+        if (phase[i] == 0) {
+          const PcgState& st = slots[i].hs;
+          if (st.done || launched[i] > o->maxiter + chunk) fail(finish_rhs(i));
+          else fail(launch_chunk(i));
+        } else {
+          tl_solve_info* inf = infos ? infos + info_at[i] + rhs[i] : nullptr;
+          if (!pcg_info(slots[i].hs, slots[i].tr2, inf)) all_conv = false;
+          if (++rhs[i] < nrhs[i]) {
+            fail(start_rhs(i));
+          } else {
+            phase[i] = 2;
+            --remaining;
+          }
+        }
+      }
+    }
+  }
+  for (int k = 0; k < nstreams; ++k)
+    if (streams[k]) (void)hipStreamSynchronize(streams[k]);
+  for (int i = 0; i < n; ++i) {
+    hs[i]->stream = saved[i];
+    hs[i]->prof_on = saved_prof[i];
+    if (ev[i]) (void)hipEventDestroy(ev[i]);
+  }
+  for (int k = 0; k < nstreams; ++k)
+    if (streams[k]) (void)hipStreamDestroy(streams[k]);
+  (void)hipHostFree(slots);
+  if (rc != TL_OK) return rc;
+  if (!all_conv) {
+    set_error("pcg iteration did not converge for at least one right-hand side");
     return TL_ENOCONV;
   }
   return TL_OK;

@@ -164,6 +164,7 @@ struct PcgState {
 };
 
 struct Level {
+  void release();
   Grid g;
   int64_t ndof = 0;
   DevBuf mask;            // uint8 per node, bit d = DOF d fixed
@@ -195,6 +196,9 @@ struct FineOp {
   bool have_density = false;
   bool fast_ke = false;        // KE matches its unit-cell pattern
   double kv[8] = {};           // group values of the pattern
+  // 2D operator workgroup shape per precision [fp64, fp32]: 0 = node line
+  // (BS consecutive nodes), TY = 16/32/64: (BS/TY) x TY node tile
+  int tile2d[2] = {0, 0};
 };
 
 }  // namespace tl
@@ -218,6 +222,12 @@ struct tl_problem {
   tl::MgState mg;
   // PCG workspace (fp64, ndof each) and reduction scratch
   tl::DevBuf r, z, p, p2, q, t;
+  tl::DevBuf xb;  // batched solves: iterate buffer captured by the chunk graph
+  // batched solves: captured chunk of PCG iterations, reused while its key
+  // (hierarchy generation, tile shapes, chunk length, preconditioner) holds
+  uint64_t gen = 0;  // bumped by every tl_setup_mg
+  hipGraphExec_t chunk_exec = nullptr;
+  int64_t chunk_key[5] = {-1, -1, -1, -1, -1};
   tl::DevBuf partials, state;
   // launch timing of the fine operator (tl_profile)
   bool prof_on = false;

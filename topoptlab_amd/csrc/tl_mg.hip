@@ -745,6 +745,7 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
   MgState& M = P->mg;
   const int ndpn = P->ndpn, nd = P->nd;
   const bool rebuild = M.nlevels != L || M.lv.empty();
+  ++P->gen;  // captured iteration chunks refer to the previous hierarchy
   // (zero-initialised gamma fields mean a V-cycle)
   TL_ARG(o->gamma >= 0 && o->gamma <= 3 && o->gamma_from >= 0, "gamma must be 0..3, gamma_from >= 0");
   M.nu1 = o->nu1; M.nu2 = o->nu2; M.omega = o->omega; M.prec = o->precision;
@@ -753,6 +754,7 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
   if (rebuild) {
     std::vector<std::vector<uint8_t>> masks;
     TL_TRY(level_masks(P, L, masks));
+    for (auto& l : M.lv) l.release();
     M.lv.clear();
     M.lv.resize(L);
     M.nlevels = L;

@@ -58,3 +58,25 @@ def seeded_loads_2d(rng, nelx, nely, ncases):
         ix, iy = cand[p]
         nodes.append((nely + 1) * ix + iy)
     return np.array(nodes, dtype=np.int64)
+
+
+def batched_instances(seed=0, n_elast=32, n_heat=8, xrange=(128, 1024), yrange=(64, 512),
+                      ncases=4, multiple=32):
+    """Instances of the batched uneven-grid workload: ``n_elast`` 2D Q4
+    elasticity grids (left edge clamped, ``ncases`` unit downward point loads
+    at seeded boundary nodes) and ``n_heat`` 2D heat-conduction grids (uniform
+    source, T = 0 on the left edge).  Element counts are drawn uniformly from
+    the ranges and rounded down to a multiple of ``multiple`` so that every
+    grid halves exactly into a multigrid hierarchy.  Returns dicts with kind,
+    nelx, nely, density_seed and (elasticity) load_nodes."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(n_elast + n_heat):
+        nelx = max(multiple, int(rng.integers(xrange[0], xrange[1] + 1)) // multiple * multiple)
+        nely = max(multiple, int(rng.integers(yrange[0], yrange[1] + 1)) // multiple * multiple)
+        d = {"kind": "elast" if i < n_elast else "heat", "nelx": nelx, "nely": nely,
+             "density_seed": int(rng.integers(0, 2 ** 31))}
+        if d["kind"] == "elast":
+            d["load_nodes"] = seeded_loads_2d(rng, nelx, nely, ncases)
+        out.append(d)
+    return out
