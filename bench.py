@@ -10,9 +10,13 @@ their compliance and displacement errors against an FP64 solve to relres
 1e-12.  Inputs (density, load vector) are resident in HBM before the timed
 region.  Prints one JSON line; extra diagnostics go to stderr.
 
+The batched uneven-grid workload (40 2D instances, 136 right-hand sides,
+tl_batch_solve) is timed after the 3D forms and reported under "batched";
+``--batched`` runs it alone.
+
 Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1] [--levels 5]
                        [--vcycle fp64|fp32|bf16|fp16] [--gamma 1] [--no-variants]
-                       [--no-cpu-baseline]
+                       [--no-cpu-baseline] [--no-batched] [--batched]
 """
 from __future__ import annotations
 
@@ -123,7 +127,7 @@ def kernel_kinds(dims, vcycle, nlevels):
     }
 
 
-def bench_batched(dev, steps=2, nstreams=4, chunk=8, cpu_sample=True):
+def bench_batched(dev, steps=2, nstreams=8, chunk=8, cpu_sample=True):
     """Batched uneven 2D solves: 32 Q4 elasticity grids (4 load cases each)
     and 8 Q4 heat grids from workloads.batched_instances(seed=0), FP64 outer
     PCG with FP32 V-cycles to relres 1e-8.  A step = set_density +
@@ -184,7 +188,7 @@ def bench_batched(dev, steps=2, nstreams=4, chunk=8, cpu_sample=True):
         "workload": "32 Q4 elasticity grids (nelx 128..1024, nely 64..512, multiples of 32, "
                     "4 point-load cases, left edge clamped, Emin 1e-9) + 8 Q4 heat grids "
                     "(kmin 1e-3, uniform source, left edge T=0); GRF densities seed 0; "
-                    "FP64 PCG, FP32 GMG V(1,1), relres 1e-8",
+                    "FP64 PCG, FP32 GMG V(1,1) (coarsest grid <= 5000 DOF), relres 1e-8",
         "instances": len(ops), "systems": len(its), "dof_total": int(sum(op.ndof for op in ops)),
         "dof_rhs_total": int(dof_rhs),
         "ms_per_step": round(ms_step, 2), "ms_setup": round(ms_setup, 2),
@@ -214,7 +218,7 @@ def bench_batched(dev, steps=2, nstreams=4, chunk=8, cpu_sample=True):
             K = fem.apply_bc(fem.assemble_K(KE, fem.edofmat_2d(sp["nelx"], sp["nely"], op.ndpn),
                                             fem.simp(inst[i][1], eps), op.ndof), free).tocsr()
             h = mg.Hierarchy(K, sp["nelx"], sp["nely"], None, op.ndpn, op.fixed,
-                             auto_levels(sp["nelx"], sp["nely"], None, op.ndpn, 1500))
+                             auto_levels(sp["nelx"], sp["nely"], None, op.ndpn, 5000))
             for k in range(inst[i][2].shape[0]):
                 mg.pcg(K, inst[i][2][k][free], h.vcycle, rtol=1e-8)
             tc += time.perf_counter() - t1
@@ -435,6 +439,12 @@ def main():
                           f"scipy sparse kernels run on 1 of {os.cpu_count()} host cores")}
         log(f"cpu baseline sample: {tc:.2f} s for {ndof_s} DOF ({itc} its)")
 
+    batched = None
+    if not args.no_batched:
+        op.close()  # the batched workload holds its own 40 handles
+        batched = bench_batched(dev, steps=max(1, args.steps - 1),
+                                cpu_sample=not args.no_cpu_baseline)
+
     dtype = {"fp32": "f32", "fp64": "f64", "bf16": "bf16-input/f32-accumulate MFMA",
              "fp16": "f16-input/f32-accumulate MFMA"}[args.vcycle]
     out = {
@@ -460,6 +470,7 @@ def main():
                    "compliance": compliance, "solve_hbm_TBps": round(solve_tbps, 3)},
         "roofline": roof,
         "variants": variants,
+        "batched": batched,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
