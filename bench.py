@@ -108,12 +108,15 @@ def kernel_kinds(dims, vcycle, nlevels):
     if mfma:
         vk = f"kucol_kernel {vcycle.upper()}-input MFMA (fp32 accumulate)"
         vbound, vpeak = "hbm", HBM_PEAK_GBS
+    elif vcycle == "fp64":
+        vk = "kucol64_kernel FP64 MFMA (v_mfma_f64_16x16x4f64)"
+        vbound, vpeak = "mfma", FP64_PEAK_TFLOPS
     else:
         vk = f"ku3d_kernel {vcycle.upper()}"
-        vbound, vpeak = "mfma", FP32_PEAK_TFLOPS if vcycle == "fp32" else FP64_PEAK_TFLOPS
+        vbound, vpeak = "mfma", FP32_PEAK_TFLOPS
     sb = 14 * 9 * nn1 * vb  # level-1 block stencil bytes (forward half: centre + 13 blocks)
     return {
-        0: ("ku3d_kernel FP64 fused PCG product (p = z + beta p; q = K p; p.q)",
+        0: ("kucol64_kernel FP64 MFMA fused PCG product (p = z + beta p; q = K p; p.q)",
             32 * ndof + 8 * nel, flops_ku, "mfma", FP64_PEAK_TFLOPS),
         1: (f"{vk} V-cycle residual (r = b - K x)",
             (2 * vb + 8) * ndof + vb * nel + ndof // 3, flops_ku, vbound, vpeak),

@@ -63,6 +63,7 @@ _SIGS = {
                                  C.POINTER(SolveOpts), C.POINTER(BatchOpts),
                                  C.POINTER(SolveInfo)]),
     "tl_set_tile2d": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32]),
+    "tl_set_fp64_operator": (C.c_int, [C.c_void_p, C.c_int32]),
     "tl_autotune": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p]),
     "tl_autotune_clear": (None, []),
     "tl_compliance_sens": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,

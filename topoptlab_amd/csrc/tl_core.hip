@@ -832,6 +832,18 @@ static int fine_apply_dispatch(tl_problem* P, KuArgs& a, int prec, int mode, boo
     return TL_OK;
   }
   if (prec == TL_PREC_FP64) {
+    if (P->op.mfma64 && P->g.dim == 3 && P->ndpn == 3 &&
+        ((mode == KU_STORE && out64) || (mode == KU_STORE_DOT && fusep) ||
+         (b64 && (mode == KU_RESID || mode == KU_JACOBI || mode == KU_JACOBI_DOT)))) {
+      unsigned nb = 0;
+      TL_TRY(fine_apply_mfma64(P, static_cast<const double*>(a.u), static_cast<double*>(a.y), mode,
+                               static_cast<const double*>(a.b), static_cast<const double*>(a.dinv),
+                               a.omega, static_cast<const double*>(a.pin),
+                               static_cast<double*>(a.pout), a.beta, a.partials, a.done, &nb));
+      if (mode == KU_STORE_DOT || mode == KU_JACOBI_DOT)
+        return launch_fold(P, nb, a.fin, a.dot_out, a.done);
+      return TL_OK;
+    }
     a.sp = P->op.sp64.ptr; a.KE = P->op.KE64.ptr;
     switch (P->ndpn) {
       case 1: return dispatch_mode<1, double>(P, a, mode, fusep, b64, out64);
@@ -1140,6 +1152,7 @@ int tl_create(const tl_grid_desc* d, tl_problem** out) {
   }
   P->fixed_h.assign(d->fixed, d->fixed + d->nfixed);
   detect_ke_pattern(P);
+  P->op.mfma64 = P->g.dim == 3 && P->ndpn == 3;
   std::vector<float> ke32(nd * nd);
   for (int i = 0; i < nd * nd; ++i) ke32[i] = float(P->KEh[i]);
   auto fail = [&](int rc) { delete P; return rc; };
@@ -1173,6 +1186,16 @@ void tl_destroy(tl_problem* p) { delete p; }
 int tl_set_stream(tl_problem* p, void* stream) {
   TL_ARG(p, "null handle");
   p->stream = static_cast<hipStream_t>(stream);
+  return TL_OK;
+}
+
+int tl_set_fp64_operator(tl_problem* p, int32_t kind) {
+  TL_ARG(p, "null handle");
+  TL_ARG(kind == 0 || kind == 1, "kind must be 0 (vector FMA) or 1 (MFMA)");
+  TL_ARG(kind == 0 || (p->g.dim == 3 && p->ndpn == 3),
+         "the FP64 MFMA operator covers 3D H8 elasticity only");
+  p->op.mfma64 = kind == 1;
+  ++p->gen;  // captured iteration chunks hold the previous kernel
   return TL_OK;
 }
 

@@ -199,6 +199,8 @@ struct FineOp {
   // 2D operator workgroup shape per precision [fp64, fp32]: 0 = node line
   // (BS consecutive nodes), TY = 16/32/64: (BS/TY) x TY node tile
   int tile2d[2] = {0, 0};
+  // FP64 3D elasticity operator: true = MFMA column walk, false = vector FMAs
+  bool mfma64 = false;
 };
 
 }  // namespace tl
@@ -250,6 +252,11 @@ int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done);
 int fine_apply_mfma(tl_problem* P, const void* u, void* y, int prec, int mode, const void* b,
                     const void* dinv, double omega, double* partials, const int* done,
                     unsigned* nblocks_out);
+// FP64 fine operator on MFMA (3D H8 elasticity)
+int fine_apply_mfma64(tl_problem* P, const double* u, double* y, int mode, const double* b,
+                      const double* dinv, double omega, const double* pin, double* pout,
+                      const double* beta, double* partials, const int* done,
+                      unsigned* nblocks_out);
 }  // namespace tl
 
 // modes of the fine matrix-free operator epilogue

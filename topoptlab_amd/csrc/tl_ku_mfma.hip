@@ -357,6 +357,254 @@ kucol_kernel(ColArgs a) {
   }
 }
 
+// ----------------------------------------------------------------------------
+// FP64 column walk on v_mfma_f64_16x16x4f64: the same walk as above with
+// double operands.  Lane layouts of the f64 MFMA: lane l holds A[l%16][l/16]
+// and B[l/16][l%16] of one K=4 slice; output element i of lane l is
+// C[l/16 + 4i][l%16].  Hence rows are ordered row = line + 4*component
+// (component 3: zero rows), so lane group kq receives node line kq's three
+// components, and K slice s (0..5) is (node gy+j, component c) with
+// s = 3j + c for input node line kq: each lane's B0 slices are its node
+// line's values at nodes gy, gy+1 times the element scale, exactly the data
+// of the BF16 lane.  B1 is B0 of column m-1 (two 32-bit DPP shifts per
+// double).  Two accumulators (elements gy and gy-1), summed in FP64.
+//
+// Modes: KU_STORE (y = K x), KU_STORE_DOT with the fused PCG update (input p = z + beta*p_in,
+// written to pout by the node's owner, q = K p, dot += p.q), KU_RESID
+// (y = b - K x), KU_JACOBI / KU_JACOBI_DOT (y = x + omega dinv (b - K x),
+// dot += b.y).  All vectors FP64.
+// ----------------------------------------------------------------------------
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+
+struct D3 {
+  double v[3];
+};
+
+__device__ inline double dpp_f64(double v, int ctrl_shr) {
+  typedef int int2v __attribute__((ext_vector_type(2)));
+  int2v w = __builtin_bit_cast(int2v, v);
+  if (ctrl_shr) {
+    w[0] = __builtin_amdgcn_update_dpp(0, w[0], 0x111 /* row_shr:1 */, 0xf, 0xf, true);
+    w[1] = __builtin_amdgcn_update_dpp(0, w[1], 0x111, 0xf, 0xf, true);
+  } else {
+    w[0] = __builtin_amdgcn_update_dpp(0, w[0], 0x101 /* row_shl:1 */, 0xf, 0xf, true);
+    w[1] = __builtin_amdgcn_update_dpp(0, w[1], 0x101, 0xf, 0xf, true);
+  }
+  return __builtin_bit_cast(double, w);
+}
+
+struct Col64Args {
+  const double* u;        // input: x (V-cycle modes) or z (fused PCG product)
+  const double* pin;      // fused product: previous search direction
+  double* pout;           // fused product: new search direction
+  const double* beta;     // fused product: device scalar
+  const double* sp;       // padded SIMP scale (fp64)
+  const double* KE;       // element matrix, fp64, 24x24 row-major
+  const uint8_t* mask;
+  const double* b;        // right-hand side (residual, Jacobi)
+  const double* dinv;     // inverse diagonal (Jacobi)
+  double omega;
+  double* y;
+  double* partials;
+  const int* done;
+  int nelx, nely, nelz, nx, ny, nz;
+  int zlen, ntx, nyc, ntask;
+};
+
+template <int TXN>
+struct Col64Layout {
+  static constexpr int NPL = TXN * CH;
+  static constexpr int PART = 3 * NPL + 8;         // doubles
+  static constexpr int HI_OFF = 2 * PART + 24;
+  static constexpr int WAVE_LDS = HI_OFF + 4 * PART;
+  static constexpr int EPR = (NPL + 63) / 64;
+};
+
+template <int MODE, int TXN>
+__global__ __launch_bounds__(BS, 2) void
+kucol64_kernel(Col64Args a) {
+  if (a.done && *a.done) return;
+  using LY = Col64Layout<TXN>;
+  constexpr int NPL64 = LY::NPL, P64 = LY::PART, EPR64 = LY::EPR;
+  constexpr bool FUSEP = MODE == KU_STORE_DOT;
+  constexpr bool DOT = MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT;
+  __shared__ double lds[WAVES * LY::WAVE_LDS];
+  __shared__ double shred[WAVES + 2];
+
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kq = lane >> 4;
+  const int dx = kq & 1, dz = kq >> 1;
+
+  const unsigned nb = gridDim.x, hw = blockIdx.x;
+  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
+  const int task = __builtin_amdgcn_readfirstlane(int(logical) * WAVES + wave);
+
+  double dot = 0.0;
+  if (task < a.ntask) {
+    const double beta = FUSEP ? *a.beta : 0.0;
+    const int cy = task % a.nyc;
+    const int rest = task / a.nyc;
+    const int tx = rest % a.ntx, cz = rest / a.ntx;
+    const int x0 = tx * TXN, y0 = cy * CH;
+    const int z0 = cz * a.zlen, z1 = min(a.nz, z0 + a.zlen);
+    const int nxl = min(TXN, a.nx - x0);
+    const int gy = y0 - 1 + m;
+    double* lw = lds + wave * LY::WAVE_LDS;
+
+    // A operands: row m -> output node line m&3 (dx_o, dz_o), component m>>2
+    // (3: zero row); K slice s of lane group kq -> input node line kq,
+    // element-local y offset s/3, component s%3
+    double A0[6], A1[6];
+    {
+      const int oline = m & 3, oc = m >> 2;
+      const int dxo = oline & 1, dzo = oline >> 1;
+#pragma unroll
+      for (int s = 0; s < 6; ++s) {
+        const int col = 3 * loc_of(dx, s / 3, dz) + s % 3;
+        A0[s] = oc < 3 ? a.KE[(3 * loc_of(dxo, 0, dzo) + oc) * 24 + col] : 0.0;
+        A1[s] = oc < 3 ? a.KE[(3 * loc_of(dxo, 1, dzo) + oc) * 24 + col] : 0.0;
+      }
+    }
+
+    const int gyn = min(max(gy, 0), a.ny - 1);
+    const int gye = min(gy, a.nely);
+    const unsigned lane_u = (__umul24(__umul24(dz, a.nx) + dx, a.ny) + gyn) * 24u;
+    const unsigned lane_s = unsigned(gye + 1) * 8u;
+    auto load_step = [&](int i, int L, D3& u3, D3& p3, double& sv) {
+      const int ex = x0 - 1 + i;
+      const int exc = min(max(ex, 0), a.nelx - 1), Lz = min(max(L, 0), a.nelz - 1);
+      const unsigned su = unsigned(Lz * a.nx + exc) * unsigned(a.ny) * 24u;
+      u3 = ldo(reinterpret_cast<const D3*>(reinterpret_cast<const char*>(a.u) + su), lane_u);
+      if constexpr (FUSEP)
+        p3 = ldo(reinterpret_cast<const D3*>(reinterpret_cast<const char*>(a.pin) + su), lane_u);
+      const int exs = min(ex, a.nelx), Ls = min(L, a.nelz);
+      const unsigned ss = unsigned((Ls + 1) * (a.nelx + 2) + exs + 1) * unsigned(a.nely + 2) * 8u;
+      sv = ldo(reinterpret_cast<const double*>(reinterpret_cast<const char*>(a.sp) + ss), lane_s);
+    };
+    unsigned eo[EPR64];
+    bool eok[EPR64];
+#pragma unroll
+    for (int k = 0; k < EPR64; ++k) {
+      const int n = lane + 64 * k, nn = min(n, NPL64 - 1);
+      const int line = nn / CH, node = nn - line * CH;
+      eok[k] = n < NPL64 && line < nxl && y0 + node < a.ny;
+      eo[k] = __umul24(min(x0 + line, a.nx - 1), a.ny) + min(y0 + node, a.ny - 1);
+    }
+    const unsigned plane = unsigned(a.nx) * unsigned(a.ny);
+
+    D3 cu[TXN + 1];
+    D3 cp[FUSEP ? TXN + 1 : 1];
+    double cs[TXN + 1];
+#pragma unroll
+    for (int i = 0; i <= TXN; ++i) load_step(i, z0 - 1, cu[i], cp[FUSEP ? i : 0], cs[i]);
+
+    for (int ez = z0 - 1; ez < z1; ++ez) {
+      const bool out_plane = ez >= z0;
+      double* lo = lw;
+      double* hi_w = lw + LY::HI_OFF + (ez & 1) * 2 * P64;
+      const double* hi_r = lw + LY::HI_OFF + ((ez + 1) & 1) * 2 * P64;
+      const unsigned zoff = unsigned(max(ez, 0)) * plane;
+      D3 eb[EPR64], ed[EPR64], exv[EPR64], epv[EPR64];
+      unsigned em[EPR64];
+#pragma unroll
+      for (int k = 0; k < EPR64; ++k) {
+        const unsigned nd = zoff + eo[k];
+        em[k] = ldo(a.mask, nd);
+        if constexpr (MODE != KU_STORE_DOT && MODE != KU_STORE)
+          eb[k] = ldo(reinterpret_cast<const D3*>(a.b), nd * 24u);
+        if constexpr (MODE == KU_JACOBI || MODE == KU_JACOBI_DOT) {
+          ed[k] = ldo(reinterpret_cast<const D3*>(a.dinv), nd * 24u);
+          exv[k] = ldo(reinterpret_cast<const D3*>(a.u), nd * 24u);
+        }
+        if constexpr (FUSEP) {
+          exv[k] = ldo(reinterpret_cast<const D3*>(a.u), nd * 24u);
+          epv[k] = ldo(reinterpret_cast<const D3*>(a.pin), nd * 24u);
+        }
+      }
+#pragma unroll
+      for (int i = 0; i <= TXN; ++i) {
+        double uv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) uv[c] = FUSEP ? fma(beta, cp[FUSEP ? i : 0].v[c], cu[i].v[c]) : cu[i].v[c];
+        const double s0 = cs[i];
+        double B0[6], B1[6];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          B0[c] = s0 * uv[c];
+          B0[3 + c] = s0 * dpp_f64(uv[c], 0);  // node gy+1 from column m+1
+        }
+#pragma unroll
+        for (int t = 0; t < 6; ++t) B1[t] = dpp_f64(B0[t], 1);  // element gy-1 = column m-1
+        load_step(i, ez + 1, cu[i], cp[FUSEP ? i : 0], cs[i]);
+        doublex4 D0 = {0.0, 0.0, 0.0, 0.0}, D1 = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int t = 0; t < 6; ++t) {
+          D0 = __builtin_amdgcn_mfma_f64_16x16x4f64(A0[t], B0[t], D0, 0, 0, 0);
+          D1 = __builtin_amdgcn_mfma_f64_16x16x4f64(A1[t], B1[t], D1, 0, 0, 0);
+        }
+        const int l = i - 1 + dx;
+        if (m >= 1 && m <= CH && l >= 0 && l < nxl) {
+          double* dst = (dz ? hi_w : lo) + dx * P64 + l * CH + (m - 1);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) dst[c * NPL64] = D0[c] + D1[c];
+        }
+      }
+      if (out_plane) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+        for (int k = 0; k < EPR64; ++k) {
+          if (!eok[k]) continue;
+          const int n = lane + 64 * k;
+          const unsigned nd = zoff + eo[k];
+          double acc[3];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const int o = c * NPL64 + n;
+            acc[c] = (lo[o] + lo[P64 + o]) + (hi_r[o] + hi_r[P64 + o]);
+          }
+          D3 out;
+          if constexpr (MODE == KU_STORE_DOT) {
+            D3 pv;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              pv.v[c] = fma(beta, epv[k].v[c], exv[k].v[c]);
+              out.v[c] = ((em[k] >> c) & 1u) ? 0.0 : acc[c];
+              dot += pv.v[c] * out.v[c];
+            }
+            sto(reinterpret_cast<D3*>(a.pout), nd * 24u, pv);
+          } else if constexpr (MODE == KU_STORE) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out.v[c] = ((em[k] >> c) & 1u) ? 0.0 : acc[c];
+          } else if constexpr (MODE == KU_RESID) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) out.v[c] = ((em[k] >> c) & 1u) ? 0.0 : eb[k].v[c] - acc[c];
+          } else {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+              out.v[c] = ((em[k] >> c) & 1u)
+                             ? 0.0
+                             : exv[k].v[c] + a.omega * ed[k].v[c] * (eb[k].v[c] - acc[c]);
+              if constexpr (MODE == KU_JACOBI_DOT) dot += eb[k].v[c] * out.v[c];
+            }
+          }
+          sto(reinterpret_cast<D3*>(a.y), nd * 24u, out);
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    }
+  }
+  if constexpr (DOT) {
+    double vv[1] = {dot};
+    block_sum<1>(vv, shred);
+    store_partials<1>(vv, a.partials);
+  }
+}
+
 int task_zlen(int ntx, int nyc, int nz) {
   // z range per task: about 4 tasks per SIMD (1024 SIMDs), at least 4 layers
   if (const char* e = std::getenv("TOPOPTLAB_AMD_MFMA_ZLEN")) {
@@ -427,6 +675,60 @@ int fine_apply_mfma(tl_problem* P, const void* u, void* y, int prec, int mode, c
     default: set_error("MFMA operator: unsupported mode"); return TL_EINVAL;
   }
 #undef TL_KUM
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+// FP64 MFMA operator (3D H8 elasticity): modes KU_STORE, KU_STORE_DOT with
+// the fused update (pin, pout, beta given), KU_RESID, KU_JACOBI,
+// KU_JACOBI_DOT.  DOT modes leave one partial per workgroup in `partials`
+// (*nblocks_out of them) for the caller's fold.
+int fine_apply_mfma64(tl_problem* P, const double* u, double* y, int mode, const double* b,
+                      const double* dinv, double omega, const double* pin, double* pout,
+                      const double* beta, double* partials, const int* done,
+                      unsigned* nblocks_out) {
+  const Grid& g = P->g;
+  TL_ARG(g.dim == 3 && P->ndpn == 3, "the MFMA operator covers 3D H8 elasticity only");
+  TL_ARG(P->op.KE64.ptr != nullptr && P->op.sp64.ptr != nullptr && P->op.mask.ptr != nullptr,
+         "internal: operator buffers missing");
+  TL_ARG(mode != KU_STORE_DOT || (pin && pout && beta), "internal: fused update operands missing");
+  TL_ARG((mode != KU_RESID && mode != KU_JACOBI && mode != KU_JACOBI_DOT) || b,
+         "internal: right-hand side missing");
+  TL_ARG((mode != KU_JACOBI && mode != KU_JACOBI_DOT) || dinv, "internal: diagonal missing");
+  constexpr int TXN = 8;
+  Col64Args a{};
+  a.u = u; a.pin = pin; a.pout = pout; a.beta = beta;
+  a.sp = P->op.sp64.as<double>();
+  a.KE = P->op.KE64.as<double>();
+  a.mask = P->op.mask.as<uint8_t>();
+  a.b = b; a.dinv = dinv; a.omega = omega; a.y = y;
+  a.partials = partials;
+  a.done = done;
+  a.nelx = g.nelx; a.nely = g.nely; a.nelz = g.nelz;
+  a.nx = g.nx; a.ny = g.ny; a.nz = g.nz;
+  a.ntx = (g.nx + TXN - 1) / TXN;
+  a.nyc = (g.ny + CH - 1) / CH;
+  a.zlen = task_zlen(a.ntx, a.nyc, g.nz);
+  const int64_t ntask = int64_t(a.ntx) * a.nyc * ((g.nz + a.zlen - 1) / a.zlen);
+  TL_ARG(ntask < (int64_t(1) << 30) && P->ndof * int64_t(sizeof(double)) < (int64_t(1) << 32) &&
+             P->g.npad() * int64_t(sizeof(double)) < (int64_t(1) << 32),
+         "grid too large for the MFMA operator (32-bit offsets)");
+  a.ntask = int(ntask);
+  const int64_t nblocks = (ntask + WAVES - 1) / WAVES;
+  const bool dotm = mode == KU_STORE_DOT || mode == KU_JACOBI_DOT;
+  TL_ARG(nblocks <= kMaxRedBlocks || !dotm, "MFMA operator grid exceeds the reduction slot count");
+  if (nblocks_out) *nblocks_out = unsigned(nblocks);
+  const dim3 grid{unsigned(nblocks), 1u, 1u}, block{unsigned(BS), 1u, 1u};
+#define TL_KUM64(MODE) hipLaunchKernelGGL((kucol64_kernel<MODE, TXN>), grid, block, 0, P->stream, a)
+  switch (mode) {
+    case KU_STORE: TL_KUM64(KU_STORE); break;
+    case KU_STORE_DOT: TL_KUM64(KU_STORE_DOT); break;
+    case KU_RESID: TL_KUM64(KU_RESID); break;
+    case KU_JACOBI: TL_KUM64(KU_JACOBI); break;
+    case KU_JACOBI_DOT: TL_KUM64(KU_JACOBI_DOT); break;
+    default: set_error("MFMA operator: unsupported mode"); return TL_EINVAL;
+  }
+#undef TL_KUM64
   TL_CHECK_LAUNCH();
   return TL_OK;
 }
