@@ -163,6 +163,15 @@ int tl_filter_hs(tl_filter* f, double* Hs);
 int tl_filter_apply(tl_filter* f, const double* in, double* out, int32_t mode,
                     const double* x);
 
+/* Node/element averaging TF of the Helmholtz PDE filter
+ * (filter/helmholtz_filter.py:223-227, 2D): mode 0: out (nodes) = TF in
+ * (elements), each node gets 1/4 of every element around it; mode 1: out
+ * (elements) = TF^T in (nodes), 1/4 of each of the element's nodes.  The
+ * filter is TF^T KF^-1 TF with KF the screened-Poisson operator, solved by
+ * tl_solve on a tl_problem made from that element matrix. */
+int tl_helmholtz_transfer(int32_t nelx, int32_t nely, const double* in, double* out,
+                          int32_t mode, void* stream);
+
 /* Optimality-criteria bisection of optimizer/optimality_criterion.py:9-79
  * (no element flags).  x, dc, dv, xnew: device n; g, gt: host scalars. */
 int tl_oc_update(int64_t n, const double* x, const double* dc,

@@ -163,6 +163,18 @@ def ke_poisson_2d(k=1.0):
                          [-2, -1, 4, -1], [-1, -2, -1, 4]], dtype=float) / 6.0
 
 
+def lm_mass_2d(p=1.0):
+    """Q4 scalar mass matrix, unit square: the closed form of
+    ``elements/mass_scalar_2d.py:68-95``."""
+    return p * np.array([[1 / 9, 1 / 18, 1 / 36, 1 / 18], [1 / 18, 1 / 9, 1 / 18, 1 / 36],
+                         [1 / 36, 1 / 18, 1 / 9, 1 / 18], [1 / 18, 1 / 36, 1 / 18, 1 / 9]])
+
+
+def ke_screened_poisson_2d(k=1.0):
+    """``elements/screenedpoisson_2d.py:9-36``: Poisson stiffness plus unit mass."""
+    return ke_poisson_2d(k) + lm_mass_2d(1.0)
+
+
 # ---------------------------------------------------------------------------
 # material interpolation, assembly, boundary conditions
 # ---------------------------------------------------------------------------
@@ -343,3 +355,21 @@ def oc_top88(x, volfrac, dc, dv, g, move=0.2, l1=0.0, l2=1e9):
         else:
             l2 = lmid
     return xnew, gt
+
+
+def assemble_helmholtz_filter(nelx, nely, rmin):
+    """``filter/helmholtz_filter.py:133-230`` (2D; the reference raises for
+    3D at 177): KF sums the screened-Poisson element matrix with
+    k = (rmin / (2 sqrt 3))^2 over all elements, no boundary conditions; TF
+    (ndof x nel) maps each element's value to its 4 nodes with weight 1/4.
+    The filter is x -> TF^T KF^-1 TF x (``helmholtz_filter.py:92``)."""
+    from scipy.sparse import coo_matrix
+    Rmin = rmin / (2.0 * np.sqrt(3.0))
+    KE = ke_screened_poisson_2d(Rmin ** 2)
+    edof = edofmat_2d(nelx, nely, 1)
+    nel = nelx * nely
+    ndof = (nelx + 1) * (nely + 1)
+    KF = assemble_K(KE, edof, np.ones(nel), ndof).tocsc()
+    TF = coo_matrix((np.full(4 * nel, 0.25), (edof.flatten(order="F"), np.tile(np.arange(nel), 4))),
+                    shape=(ndof, nel)).tocsc()
+    return KF, TF

@@ -2,20 +2,21 @@
 
 Test infrastructure only (see ``oracle/__init__.py``).  Restates the body of
 ``topoptlab/topology_optimization.py:main`` (loop at 411-673) for the branch
-``optimizer="oc"``, ``filter_mode="matrix"``, ``ft in {-1, 0, 1}``,
+``optimizer="oc"``, ``filter_mode`` "matrix" or "helmholtz" (2D; branches at
+360-365, 554-555, 568-570, 646-647), ``ft in {-1, 0, 1}``,
 ``lin_solver_kw={"name": "scipy-direct"}``, compliance objective, SIMP.
 """
 from __future__ import annotations
 
 import numpy as np
-from scipy.sparse.linalg import spsolve
+from scipy.sparse.linalg import factorized, spsolve
 
 from . import fem
 
 
 def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
                 KE=None, eps=1e-9, penal=3.0, nouteriter=2000,
-                stop_on_change=True, history=None):
+                stop_on_change=True, history=None, filter_mode="matrix"):
     """Run the OC loop; return (x, obj) like ``topology_optimization.py:688``.
 
     ``history`` (a list) receives one (obj, vol, change) tuple per iteration,
@@ -34,8 +35,14 @@ def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
     x = volfrac * np.ones(n)
     xPhys = x.copy()
     g = 0.0
-    H, Hs = fem.assemble_matrix_filter(nelx, nely, rmin, nelz)
-    Hs = Hs[:, 0]
+    helm = filter_mode == "helmholtz"
+    if helm:
+        KF, TF = fem.assemble_helmholtz_filter(nelx, nely, rmin)
+        lu_solve = factorized(KF)
+        hfilt = lambda v: TF.T @ lu_solve(TF @ v)
+    else:
+        H, Hs = fem.assemble_matrix_filter(nelx, nely, rmin, nelz)
+        Hs = Hs[:, 0]
     if ndim == 2:
         f, fixed, free = bcs(nelx, nely, ndof)
     else:
@@ -50,14 +57,21 @@ def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
         u[free, :] = sol.reshape(len(free), -1)
         obj, dc = fem.compliance_sens(KE, edof, u, xPhys, eps, penal)
         dv = np.ones(n)
-        if ft == 0:
+        if ft == 0 and helm:
+            dc = hfilt(dc * xPhys) / np.maximum(0.001, x)
+        elif ft == 1 and helm:
+            dc = hfilt(dc)
+            dv = hfilt(dv)
+        elif ft == 0:
             dc = (H @ (x * dc)) / Hs / np.maximum(0.001, x)
         elif ft == 1:
             dc = H @ (dc / Hs)
             dv = H @ (dv / Hs)
         xold = x.copy()
         x, g = fem.oc_top88(x, volfrac, dc, dv, g)
-        if ft == 1:
+        if ft == 1 and helm:
+            xPhys = hfilt(x)
+        elif ft == 1:
             xPhys = (H @ x) / Hs
         else:
             xPhys = x.copy()

@@ -226,6 +226,35 @@ def test_main_mbb_golden(gpu, kats, ft, ref_idx):
     assert obj == pytest.approx(ref, abs=1.5e-5)
 
 
+@pytest.mark.parametrize("ft,ref_idx", [(0, 2), (1, 3)])
+def test_main_helmholtz_golden(gpu, kats, ft, ref_idx):
+    """tests/test_compliance.py:11-12 (Helmholtz filter) on the device path."""
+    from topoptlab_amd.topology_optimization import main
+    say("main helmholtz golden", ft)
+    ref = kats["compliance_filters"]["rows"][ref_idx][-1]
+    x, obj = main(60, 20, 0.5, 2.4, ft=ft, filter_mode="helmholtz",
+                  lin_solver_kw={"name": "amd-gmgpcg", "rtol": 1e-12}, device=gpu)
+    assert x.shape == (1200, 1)
+    assert obj == pytest.approx(ref, abs=1.5e-5)
+
+
+@pytest.mark.parametrize("nelx,nely", [(60, 20), (37, 13)])
+def test_helmholtz_filter_matches_oracle(gpu, nelx, nely):
+    import torch
+    from scipy.sparse.linalg import spsolve
+    from topoptlab_amd.filter import HelmholtzFilter
+    say("helmholtz filter", nelx, nely)
+    KF, TF = fem.assemble_helmholtz_filter(nelx, nely, 2.4)
+    x = np.random.default_rng(nelx).random(nelx * nely)
+    ref = TF.T @ spsolve(KF, TF @ x)
+    H = HelmholtzFilter(nelx, nely, 2.4, device=gpu)
+    y = H.apply_filter(torch.as_tensor(x, device=gpu)).cpu().numpy()
+    assert rel(y, ref) < 1e-10
+    assert np.abs(H.apply_filter(torch.full((nelx * nely,), 0.3, dtype=torch.float64,
+                                            device=gpu)).cpu().numpy() - 0.3).max() < 1e-10
+    H.close()
+
+
 def test_main_twoloads_golden(gpu, kats):
     from topoptlab_amd.topology_optimization import main
     say("main twoloads golden")

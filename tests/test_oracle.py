@@ -249,15 +249,35 @@ def test_c2_density_field():
 
 
 # ---------------------------------------------------------------- end-to-end goldens
-@pytest.mark.parametrize("case", [0, 1, 4, 5])
+@pytest.mark.parametrize("case", [0, 1, 2, 3, 4, 5])
 def test_compliance_goldens(kats, case):
-    """tests/test_compliance.py:8-35 (matrix filter rows), 5 decimals."""
+    """tests/test_compliance.py:8-35 (matrix and Helmholtz filter rows), 5 decimals."""
     nelx, nely, vf, ft, rmin, mode, bcname, ref = kats["compliance_filters"]["rows"][case]
-    assert mode == "matrix"
+    assert mode in ("matrix", "helmholtz")
     bcs = {"mbb_2d": fem.mbb_2d, "cantilever_2d": fem.cantilever_2d,
            "cantilever_2d_twoloads": fem.cantilever_2d_twoloads}[bcname]
-    _, obj = topopt.topopt_loop(nelx, nely, vf, rmin, ft=ft, bcs=bcs)
+    _, obj = topopt.topopt_loop(nelx, nely, vf, rmin, ft=ft, bcs=bcs, filter_mode=mode)
     assert obj == pytest.approx(ref, abs=1.5e-5)
+
+
+def test_helmholtz_filter_properties():
+    """The Helmholtz filter TF^T KF^-1 TF keeps constants exactly (TF 1 is
+    the lumped mass M 1 and KF 1 = M 1 because the Laplacian part vanishes on
+    constants), is symmetric, and its element matrices agree with the device
+    side's quadrature (mass by 2x2 Gauss, Poisson part by gradients)."""
+    from scipy.sparse.linalg import spsolve
+    from topoptlab_amd.elements import lk_screened_poisson_2d, lm_mass_2d
+    KF, TF = fem.assemble_helmholtz_filter(13, 7, 2.4)
+    F = lambda v: TF.T @ spsolve(KF, TF @ v)
+    assert np.abs(F(np.full(91, 0.42)) - 0.42).max() < 1e-14
+    rng = np.random.default_rng(0)
+    a, b = rng.standard_normal(91), rng.standard_normal(91)
+    assert abs(F(a) @ b - a @ F(b)) < 1e-13 * np.linalg.norm(a) * np.linalg.norm(b)
+    assert np.abs(lm_mass_2d() - fem.lm_mass_2d()).max() < 1e-15
+    k = (2.4 / (2 * np.sqrt(3))) ** 2
+    assert np.abs(lk_screened_poisson_2d(k=k) - fem.ke_screened_poisson_2d(k)).max() < 1e-15
+    # a filter that did not normalise (TF without the 1/4) would scale constants by 4
+    assert np.abs(TF.sum(axis=0) - 1.0).max() < 1e-15
 
 
 def test_heat_golden_loose(kats):

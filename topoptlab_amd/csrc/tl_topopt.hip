@@ -294,6 +294,32 @@ Scratch& scratch_for_device() {
 
 using namespace tl;
 
+__global__ void helmholtz_transfer_kernel(int nelx, int nely, const double* __restrict__ in,
+                                          double* __restrict__ out, int mode) {
+  const int ny = nely + 1;
+  const int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (mode == 0) {
+    if (i >= int64_t(nelx + 1) * ny) return;
+    const int ix = int(i / ny), iy = int(i % ny);
+    double acc = 0.0;
+    for (int ex = ix - 1; ex <= ix; ++ex)
+      for (int ey = iy - 1; ey <= iy; ++ey)
+        if (ex >= 0 && ex < nelx && ey >= 0 && ey < nely) acc += 0.25 * in[int64_t(ex) * nely + ey];
+    out[i] = acc;
+  } else {
+    if (i >= int64_t(nelx) * nely) return;
+    const int ex = int(i / nely), ey = int(i % nely);
+    // local node order (BL, BR, TR, TL) = nodes (ex, ey+1), (ex+1, ey+1), (ex+1, ey), (ex, ey)
+    const int64_t n1 = int64_t(ex) * ny + ey, n2 = int64_t(ex + 1) * ny + ey;
+    double acc = 0.0;
+    acc += 0.25 * in[n1 + 1];
+    acc += 0.25 * in[n2 + 1];
+    acc += 0.25 * in[n2];
+    acc += 0.25 * in[n1];
+    out[i] = acc;
+  }
+}
+
 extern "C" {
 
 int tl_filter_create(int32_t ndim, int32_t nelx, int32_t nely, int32_t nelz, double rmin,
@@ -468,4 +494,21 @@ int tl_mean_maxdiff(int64_t n, const double* a, const double* b, double* mean, d
   return TL_OK;
 }
 
+
+// Node <-> element averaging of the Helmholtz filter (TF of
+// filter/helmholtz_filter.py:223-227, weight 1/4 per element node).
+// mode 0: node[n] = 1/4 * sum of the (up to 4) elements around node n, in
+// ascending element order (TF x); mode 1: el[e] = 1/4 * sum of its 4 nodes in
+// local order (TF^T y).
+int tl_helmholtz_transfer(int32_t nelx, int32_t nely, const double* in, double* out, int32_t mode,
+                          void* stream) {
+  TL_ARG(in && out && nelx >= 1 && nely >= 1, "null argument or empty grid");
+  TL_ARG(mode == 0 || mode == 1, "mode must be 0 (TF x) or 1 (TF^T y)");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const int64_t n = mode == 0 ? int64_t(nelx + 1) * (nely + 1) : int64_t(nelx) * nely;
+  hipLaunchKernelGGL(helmholtz_transfer_kernel, dim3(unsigned(nblk(n))), dim3(BS), 0, s, nelx,
+                     nely, in, out, mode);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
 }  // extern "C"

@@ -132,6 +132,28 @@ def lk_poisson_2d(k=1.0, l=np.array([1.0, 1.0]), g=np.array([0.0]), t=1.0, **kwa
     return t * k * K
 
 
+def lm_mass_2d(p=1.0, l=np.array([1.0, 1.0]), t=1.0, **kwargs):
+    """Q4 scalar mass matrix p * int N N^T (elements/mass_scalar_2d.py:68-95),
+    by 2x2 Gauss quadrature (exact for the biquadratic products)."""
+    _check_unit(l, np.array([0.0]), 2)
+    offs = LOCAL_OFFSETS_2D
+    sgn = 2 * offs - 1
+    sgn[:, 1] *= -1
+    gp = 0.5 / np.sqrt(3.0)
+    M = np.zeros((4, 4))
+    for q in itertools.product((-gp, gp), repeat=2):
+        N = np.prod(0.5 + sgn * np.array(q)[None, :], axis=1)
+        M += 0.25 * np.outer(N, N)
+    return p * t * M
+
+
+def lk_screened_poisson_2d(k=1.0, l=np.array([1.0, 1.0]), g=np.array([0.0]), t=1.0, **kwargs):
+    """Screened Poisson element matrix k * Poisson + mass
+    (elements/screenedpoisson_2d.py:9-36); k is the squared filter radius in
+    the Helmholtz filter."""
+    return lk_poisson_2d(k=k, l=l, g=g, t=t) + lm_mass_2d(p=1.0, l=l, t=t)
+
+
 def lk_poisson_3d(k=1.0, l=np.array([1.0, 1.0, 1.0]), g=np.array([0.0, 0.0]), **kwargs):
     _check_unit(l, g, 3)
     K = np.zeros((8, 8))

@@ -81,3 +81,75 @@ def assemble_matrix_filter(nelx: int, nely: int, rmin: float, nelz: Optional[int
         raise NotImplementedError("only a scalar rmin is supported")
     H = DeviceFilter(nelx, nely, float(rmin), nelz, device)
     return H, H.Hs[:, None]
+
+
+class HelmholtzFilter:
+    """The Helmholtz PDE filter of topoptlab/filter/helmholtz_filter.py:16-130
+    (Lazarov & Sigmund 2011, in the form of top88), on the GPU:
+    x_filtered = TF^T KF^-1 TF x.
+
+    KF is the screened-Poisson operator (k = (rmin / (2 sqrt 3))^2, no
+    boundary conditions) applied matrix-free by a unit-scale scalar Q4
+    ``StiffnessOperator``; KF^-1 is a PCG solve to ``rtol`` (the reference
+    factorises KF with a sparse LU).  TF averages each element onto its 4
+    nodes (``tl_helmholtz_transfer``).  2D only, like the reference (its 3D
+    branch raises at helmholtz_filter.py:177)."""
+
+    def __init__(self, nelx: int, nely: int, rmin: float, nelz: Optional[int] = None,
+                 l=None, rtol: float = 1e-12, device=None, **kwargs):
+        import numpy as np
+        from .elements import lk_screened_poisson_2d
+        from .fem import StiffnessOperator, auto_levels
+        if nelz is not None:
+            raise NotImplementedError("3D not yet completely implemented.")
+        if l is not None and not np.allclose(np.asarray(l, dtype=float), 1.0):
+            raise NotImplementedError("only unit elements are supported")
+        self.nelx, self.nely = int(nelx), int(nely)
+        self.nel = self.nelx * self.nely
+        self.rtol = float(rtol)
+        self.device = torch.device("cuda", torch.cuda.current_device()) if device is None \
+            else torch.device(device)
+        Rmin = float(rmin) / (2.0 * np.sqrt(3.0))
+        KE = lk_screened_poisson_2d(k=Rmin ** 2)
+        # unit scale: eps = 0, penal = 1, density 1 gives s = 1 exactly
+        self.KF = StiffnessOperator(self.nelx, self.nely, KE=KE, fixed=(), eps=0.0, penal=1.0,
+                                    device=self.device)
+        self.KF.set_density(torch.ones(self.nel, dtype=torch.float64, device=self.device))
+        levels = auto_levels(self.nelx, self.nely, None, 1, 2000)
+        self._prec = "gmg" if levels >= 2 else "jacobi"
+        if levels >= 2:
+            self.KF.setup_mg(nlevels=levels, precision="fp64")
+
+    def close(self):
+        if getattr(self, "KF", None) is not None:
+            self.KF.close()
+            self.KF = None
+
+    def _transfer(self, v, mode):
+        n_in = self.nel if mode == 0 else self.KF.ndof
+        n_out = self.KF.ndof if mode == 0 else self.nel
+        v = torch.as_tensor(v, dtype=torch.float64, device=self.device).reshape(-1).contiguous()
+        if v.numel() != n_in:
+            raise ValueError(f"vector of length {v.numel()}, expected {n_in}")
+        out = torch.empty(n_out, dtype=torch.float64, device=self.device)
+        check(lib().tl_helmholtz_transfer(self.nelx, self.nely, C.c_void_p(v.data_ptr()),
+                                          C.c_void_p(out.data_ptr()), int(mode),
+                                          C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)),
+              "tl_helmholtz_transfer")
+        return out
+
+    def _filter(self, v):
+        shape = v.shape if isinstance(v, torch.Tensor) else None
+        xn = self._transfer(v, 0)
+        y, _ = self.KF.solve(xn, rtol=self.rtol, preconditioner=self._prec)
+        out = self._transfer(y, 1)
+        return out.reshape(shape) if shape is not None and len(shape) == 2 else out
+
+    def apply_filter(self, x, **kwargs):
+        """TF^T KF^-1 TF x (helmholtz_filter.py:67-92)."""
+        return self._filter(x)
+
+    def apply_filter_dx(self, x_filtered, dx_filtered, **kwargs):
+        """The same operator on the sensitivities (helmholtz_filter.py:94-130;
+        the filter is symmetric)."""
+        return self._filter(dx_filtered)

@@ -24,7 +24,7 @@ import torch
 from .elements import lk_linear_elast_2d, lk_linear_elast_3d
 from .example_bc import mbb_2d
 from .fem import StiffnessOperator
-from .filter import DeviceFilter
+from .filter import DeviceFilter, HelmholtzFilter
 from .optimizer import mean_maxdiff, oc_top88
 
 LOG_LINE = "it.: {0} obj.: {1:.10f} vol.: {2:.10f} ch.: {3:.10f}"
@@ -55,8 +55,9 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
         raise NotImplementedError(f"optimizer {optimizer!r} is not on the device path")
     if ft not in (-1, 0, 1):
         raise NotImplementedError(f"filter {ft!r} is not on the device path")
-    if filter_mode not in ("matrix", "convolution"):
+    if filter_mode not in ("matrix", "convolution", "helmholtz"):
         raise NotImplementedError(f"filter_mode {filter_mode!r} is not on the device path")
+    helm = filter_mode == "helmholtz" and ft in (0, 1)
     if body_forces_kw or el_flags is not None or mix is not None:
         raise NotImplementedError("body forces, element flags and mixing are not supported")
     if (accelerator_kw or {}).get("accelerator") is not None:
@@ -95,7 +96,10 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
             log.addHandler(logging.FileHandler(output_kw["file"] + ".log", mode="w"))
         log.setLevel(output_kw.get("verbosity", 20))
     op = StiffnessOperator(nelx, nely, nelz, KE, fixed, eps, penal, device=dev)
-    H = DeviceFilter(nelx, nely, rmin, nelz, device=dev) if ft in (0, 1) else None
+    if helm:  # topology_optimization.py:360-365
+        H = HelmholtzFilter(nelx, nely, rmin, nelz, device=dev)
+    else:
+        H = DeviceFilter(nelx, nely, rmin, nelz, device=dev) if ft in (0, 1) else None
     f = torch.as_tensor(np.ascontiguousarray(f_h.T), dtype=torch.float64, device=dev)  # [nbc][ndof]
     ncases = f.shape[0]
     u = torch.zeros_like(f)
@@ -103,7 +107,10 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
     xPhys = x.clone()
     g = 0.0
     dv0 = torch.ones(n, dtype=torch.float64, device=dev)
-    dv_f = H.apply(dv0, 1) if ft == 1 else dv0
+    if helm:
+        dv_f = H.apply_filter(dv0) if ft == 1 else dv0
+    else:
+        dv_f = H.apply(dv0, 1) if ft == 1 else dv0
     obj = 0.0
     for loop in range(nouteriter):
         op.set_density(xPhys)
@@ -113,13 +120,20 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
         for i in range(ncases):
             u[i], _ = op.solve(f[i], u0=u[i], preconditioner=prec, **solver)
         obj, dc = op.compliance_sens(u, xPhys)
-        if ft == 0:
+        if helm and ft == 0:  # topology_optimization.py:554-555
+            dc = H.apply_filter(dc * xPhys) / torch.clamp(x, min=0.001)
+        elif helm:  # 568-570
+            dc = H.apply_filter(dc)
+        elif ft == 0:
             dc = H.apply(dc, 2, x=x)
         elif ft == 1:
             dc = H.apply(dc, 1)
         xold = x
         x, g = oc_top88(x, volfrac, dc, dv_f, g)
-        xPhys = H.apply(x, 0) if ft == 1 else x.clone()
+        if helm:  # 646-647
+            xPhys = H.apply_filter(x) if ft == 1 else x.clone()
+        else:
+            xPhys = H.apply(x, 0) if ft == 1 else x.clone()
         vol, change = mean_maxdiff(xPhys, None)[0], mean_maxdiff(x, xold)[1]
         if history is not None:
             history.append((obj, vol, change))
