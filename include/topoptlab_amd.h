@@ -240,6 +240,26 @@ int tl_nn_prolong(tl_problem* p, const float* y, double* x);
  * work: ndof float64; *alpha returned on the host (synchronises). */
 int tl_energy_scale(tl_problem* p, const double* f, double* x, double* work, double* alpha);
 
+/* Method of moving asymptotes for one constraint: the mmapy.mmasub step of
+ * topology_optimization.py:591-607 with the defaults of
+ * optimizer/mma_utils.py:8-60 (Svanberg's published algorithm; mmapy is not
+ * vendored).  m = 1: a, c, d are scalars. */
+typedef struct {
+  double move, asyinit, asydecr, asyincr, asymin, asymax, raa0, albefa;
+  double a0, a, c, d;
+} tl_mma_opts;
+/* x (device n): xval on entry, xmma on exit.  low, upp (device n): the
+ * asymptotes, updated.  xold1, xold2 (device n): the previous two iterates,
+ * read when iter > 2 (the asymptotes restart at x -/+ asyinit*(xmax-xmin)
+ * for iter <= 2).  xmin, xmax (device n) may be NULL: 0 and 1.  df0dx, dfdx
+ * (device n): objective and constraint gradients; fval: constraint value.
+ * *newton_its (may be NULL): Newton steps of the subproblem.  Synchronises
+ * the stream (the subproblem's scalars live on the host). */
+int tl_mma_update(int64_t n, double* x, const double* xold1, const double* xold2,
+                  const double* xmin, const double* xmax, const double* df0dx, double fval,
+                  const double* dfdx, double* low, double* upp, int32_t iter,
+                  const tl_mma_opts* opts, void* stream, int32_t* newton_its);
+
 /* mean(a) and max|a-b| of device vectors (topology_optimization.py:516,654). */
 int tl_mean_maxdiff(int64_t n, const double* a, const double* b, double* mean,
                     double* maxdiff, void* stream);

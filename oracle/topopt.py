@@ -11,12 +11,12 @@ from __future__ import annotations
 import numpy as np
 from scipy.sparse.linalg import factorized, spsolve
 
-from . import fem
+from . import fem, mma
 
 
 def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
                 KE=None, eps=1e-9, penal=3.0, nouteriter=2000,
-                stop_on_change=True, history=None, filter_mode="matrix"):
+                stop_on_change=True, history=None, filter_mode="matrix", optimizer="oc"):
     """Run the OC loop; return (x, obj) like ``topology_optimization.py:688``.
 
     ``history`` (a list) receives one (obj, vol, change) tuple per iteration,
@@ -49,6 +49,9 @@ def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
         f, fixed, free = bcs(nelx, nely, nelz, ndof)
     u = np.zeros(f.shape)
     xold = x.copy()
+    if optimizer == "mma":  # topology_optimization.py:244-246, mma_utils.py:8-60
+        kw = mma.mma_defaultkws(n, 1, ft)
+        xhist = [x[:, None].copy() for _ in range(3)]
     obj = 0.0
     for loop in range(nouteriter):
         scale = fem.simp(xPhys, eps, penal)
@@ -56,7 +59,9 @@ def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
         sol = spsolve(K, f[free])
         u[free, :] = sol.reshape(len(free), -1)
         obj, dc = fem.compliance_sens(KE, edof, u, xPhys, eps, penal)
-        dv = np.ones(n)
+        # volume constraint and its gradient (topology_optimization.py:513-520)
+        dv = np.full(n, 1.0 / n) if optimizer == "mma" else np.ones(n)
+        constr = xPhys.mean() - volfrac
         if ft == 0 and helm:
             dc = hfilt(dc * xPhys) / np.maximum(0.001, x)
         elif ft == 1 and helm:
@@ -68,7 +73,19 @@ def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
             dc = H @ (dc / Hs)
             dv = H @ (dv / Hs)
         xold = x.copy()
-        x, g = fem.oc_top88(x, volfrac, dc, dv, g)
+        if optimizer == "mma":
+            # iter=0: the call site's leftover load-case index (see oracle/mma.py)
+            out = mma.mmasub(1, n, 0, x[:, None], kw["xmin"], kw["xmax"], xhist[-1], xhist[-2],
+                             obj, dc[:, None], np.array([[constr]]), dv[None, :], kw["low"],
+                             kw["upp"], kw["a0"], kw["a"], kw["c"], kw["d"], move=kw["move"],
+                             asyinit=kw["asyinit"], asydecr=kw["asydecr"], asyincr=kw["asyincr"],
+                             asymin=kw["asymin"], asymax=kw["asymax"], raa0=kw["raa0"],
+                             albefa=kw["albefa"])
+            kw["low"], kw["upp"] = out[9], out[10]
+            x = out[0][:, 0].copy()
+            xhist = (xhist + [x[:, None].copy()])[-4:]
+        else:
+            x, g = fem.oc_top88(x, volfrac, dc, dv, g)
         if ft == 1 and helm:
             xPhys = hfilt(x)
         elif ft == 1:

@@ -3,7 +3,8 @@
 Reads (as text, with ``ast``; nothing is imported or executed) the literal
 known-answer data of
   tests/test_gmg.py:8-170        coarse DOF index arrays (create_coarse_inds)
-  tests/test_compliance.py:8-113 end-to-end compliance goldens
+  tests/test_compliance.py:8-113 end-to-end compliance goldens (filters,
+                                 optimizers, heat)
   tests/test_linsolver.py:10-28  the 3x3 linear-system known answer
 and stores it as JSON.  Run once in a container that has the reference at
 /root/reference; the JSON is committed, the GPU box never needs the reference.
@@ -51,6 +52,11 @@ def main():
     names, rows = _parametrize_rows(REF / "test_compliance.py", "test_compliance_filters")
     out["compliance_filters"] = {"names": names, "rows": rows,
                                  "source": "tests/test_compliance.py:7-35 (assert to 5 decimals)"}
+    names, rows = _parametrize_rows(REF / "test_compliance.py", "test_compliance_optimizers")
+    out["compliance_optimizers"] = {
+        "names": names, "rows": rows,
+        "case": "nelx 60, nely 20, volfrac 0.5, ft 1, rmin 2.4, matrix filter, mbb_2d",
+        "source": "tests/test_compliance.py:62-86 (assert to 5 decimals)"}
     names, rows = _parametrize_rows(REF / "test_compliance.py", "test_compliance_heat")
     out["compliance_heat"] = {"names": names, "rows": rows,
                               "source": "tests/test_compliance.py:88-113 (3 decimals)"}

@@ -394,3 +394,46 @@ def test_fp64_mfma_operator_matches_vector(gpu, dims):
         assert abs(res[1][1] - res[0][1]) <= 1
         assert rel(res[1][0], res[0][0]) < 1e-7
     op.close()
+
+
+@pytest.mark.parametrize("it", [0, 5])
+def test_mma_update_matches_oracle(gpu, it):
+    """tl_mma_update against the oracle's mmasub: both asymptote branches."""
+    import torch
+    from oracle import mma
+    from topoptlab_amd.optimizer import mma_defaultkws, mma_update
+    say("mma update", it)
+    rng = np.random.default_rng(it)
+    n = 3000
+    x = 0.05 + 0.9 * rng.random(n)
+    xold1 = np.clip(x + 0.05 * rng.standard_normal(n), 0.0, 1.0)
+    xold2 = np.clip(xold1 + 0.05 * rng.standard_normal(n), 0.0, 1.0)
+    df0 = -rng.random(n) * 10.0
+    dfdx = np.full(n, 1.0 / n)
+    fval = 0.013
+    low0, upp0 = x - 0.2 - 0.1 * rng.random(n), x + 0.2 + 0.1 * rng.random(n)
+    okw = mma.mma_defaultkws(n, 1)
+    out = mma.mmasub(1, n, it, x[:, None], okw["xmin"], okw["xmax"], xold1[:, None],
+                     xold2[:, None], 0.0, df0[:, None], np.array([[fval]]), dfdx[None, :],
+                     low0[:, None], upp0[:, None], okw["a0"], okw["a"], okw["c"], okw["d"],
+                     move=okw["move"])
+    kw = mma_defaultkws(n, 1, device=gpu)
+    kw["low"] = torch.as_tensor(low0, device=gpu)
+    kw["upp"] = torch.as_tensor(upp0, device=gpu)
+    t = lambda a: torch.as_tensor(a, device=gpu)
+    xd = mma_update(t(x), t(xold1), t(xold2), t(df0), fval, t(dfdx), kw, iter=it)
+    assert rel(kw["low"].cpu().numpy(), out[9][:, 0]) < 1e-15
+    assert rel(kw["upp"].cpu().numpy(), out[10][:, 0]) < 1e-15
+    assert rel(xd.cpu().numpy(), out[0][:, 0]) < 1e-9
+    assert kw["newton_its"] > 0
+
+
+def test_main_mma_golden(gpu, kats):
+    """tests/test_compliance.py:62-86 (MMA) on the device path, 5 decimals."""
+    from topoptlab_amd.topology_optimization import main
+    say("main mma golden")
+    (opt, ref), = kats["compliance_optimizers"]["rows"]
+    x, obj = main(60, 20, 0.5, 2.4, ft=1, optimizer="mma",
+                  lin_solver_kw={"name": "amd-gmgpcg", "rtol": 1e-12}, device=gpu)
+    assert x.shape == (1200, 1)
+    assert obj == pytest.approx(ref, abs=1.5e-5)

@@ -287,3 +287,59 @@ def test_heat_golden_loose(kats):
     ref = kats["compliance_heat"]["rows"][0]
     _, obj = topopt.topopt_loop(40, 40, 0.4, 1.2, ft=0, bcs=fem.heatplate_2d, KE=fem.ke_poisson_2d())
     assert obj == pytest.approx(ref, rel=3e-3)
+
+
+# ---------------------------------------------------------------- MMA
+def test_mma_golden(kats):
+    """tests/test_compliance.py:62-86: MBB 60x20, ft=1, MMA, 5 decimals.  The
+    golden depends on the call site passing iter=0 (asymptotes restart each
+    iteration): a true iteration counter ends at 233.76998 instead."""
+    (opt, ref), = kats["compliance_optimizers"]["rows"]
+    assert opt == "mma"
+    _, obj = topopt.topopt_loop(60, 20, 0.5, 2.4, ft=1, optimizer="mma")
+    assert obj == pytest.approx(ref, abs=1.5e-5)
+
+
+def test_mma_subproblem_matches_slsqp():
+    """The primal-dual subproblem solver against scipy's SLSQP on the same
+    convex separable approximation (min sum p0/(u-x) + q0/(x-l) subject to
+    sum P/(u-x) + Q/(x-l) <= b, alfa <= x <= beta; with c = 1e4 the
+    artificial y, z vanish), for both asymptote branches."""
+    from scipy.optimize import minimize
+    from oracle import mma
+    rng = np.random.default_rng(1)
+    n = 8
+    kw = mma.mma_defaultkws(n, 1)
+    x = 0.2 + 0.6 * rng.random((n, 1))
+    xold1 = np.clip(x + 0.05 * rng.standard_normal((n, 1)), 0.01, 0.99)
+    xold2 = np.clip(xold1 + 0.05 * rng.standard_normal((n, 1)), 0.01, 0.99)
+    df0 = -rng.random((n, 1)) - 0.1
+    dfdx = np.full((1, n), 1.0 / n)
+    fval = np.array([[0.02]])
+    low, upp = x - 0.3, x + 0.4
+    for it in (0, 5):
+        out = mma.mmasub(1, n, it, x, kw["xmin"], kw["xmax"], xold1, xold2, 0.0, df0, fval, dfdx,
+                         low, upp, kw["a0"], kw["a"], kw["c"], kw["d"], move=kw["move"])
+        xm, lo, up = out[0][:, 0], out[9][:, 0], out[10][:, 0]
+        # rebuild the approximation the subproblem solved
+        xv = x[:, 0]
+        ux, xl = up - xv, xv - lo
+        p0 = np.maximum(df0[:, 0], 0); q0 = np.maximum(-df0[:, 0], 0)
+        pq = 0.001 * (p0 + q0) + 1e-5
+        p0, q0 = (p0 + pq) * ux ** 2, (q0 + pq) * xl ** 2
+        P = np.maximum(dfdx[0], 0); Q = np.maximum(-dfdx[0], 0)
+        PQ = 0.001 * (P + Q) + 1e-5
+        P, Q = (P + PQ) * ux ** 2, (Q + PQ) * xl ** 2
+        b = (P / ux + Q / xl).sum() - fval[0, 0]
+        alfa = np.maximum(np.maximum(lo + 0.1 * (xv - lo), xv - 0.2), 0.0)
+        beta = np.minimum(np.minimum(up - 0.1 * (up - xv), xv + 0.2), 1.0)
+        res = minimize(lambda t: (p0 / (up - t) + q0 / (t - lo)).sum(), 0.5 * (alfa + beta),
+                       jac=lambda t: p0 / (up - t) ** 2 - q0 / (t - lo) ** 2, method="SLSQP",
+                       bounds=list(zip(alfa, beta)),
+                       constraints=[{"type": "ineq",
+                                     "fun": lambda t: b - (P / (up - t) + Q / (t - lo)).sum(),
+                                     "jac": lambda t: -(P / (up - t) ** 2 - Q / (t - lo) ** 2)}],
+                       options={"ftol": 1e-14, "maxiter": 500})
+        assert res.success
+        assert np.abs(xm - res.x).max() < 1e-5, it
+        assert np.all(xm >= alfa - 1e-12) and np.all(xm <= beta + 1e-12)

@@ -40,6 +40,11 @@ class BatchOpts(C.Structure):
     _fields_ = [("nstreams", C.c_int32), ("chunk", C.c_int32), ("use_graphs", C.c_int32)]
 
 
+class MmaOpts(C.Structure):
+    _fields_ = [(k, C.c_double) for k in ("move", "asyinit", "asydecr", "asyincr", "asymin",
+                                          "asymax", "raa0", "albefa", "a0", "a", "c", "d")]
+
+
 class SolveInfo(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("status", C.c_int32),
                 ("relres_recursive", C.c_double), ("relres_true", C.c_double)]
@@ -96,6 +101,10 @@ _SIGS = {
     "tl_nn_inject": (C.c_int, [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]),
     "tl_nn_prolong": (C.c_int, [C.c_void_p] * 3),
     "tl_energy_scale": (C.c_int, [C.c_void_p] * 4 + [C.POINTER(C.c_double)]),
+    "tl_mma_update": (C.c_int, [C.c_int64] + [C.c_void_p] * 6 + [C.c_double, C.c_void_p,
+                                                                   C.c_void_p, C.c_void_p,
+                                                                   C.c_int32, C.POINTER(MmaOpts),
+                                                                   C.c_void_p, C.c_void_p]),
     "tl_mean_maxdiff": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.POINTER(C.c_double),
                                   C.POINTER(C.c_double), C.c_void_p]),
 }

@@ -1,4 +1,4 @@
-"""Optimality-criteria update on the GPU.
+"""Optimality-criteria and MMA updates on the GPU.
 
 ``oc_top88`` keeps the signature of
 topoptlab/optimizer/optimality_criterion.py:9-79 (Nguyen/Paulino top88
@@ -12,6 +12,7 @@ import ctypes as C
 
 import torch
 
+from . import _lib
 from ._lib import check, lib
 
 
@@ -45,3 +46,46 @@ def mean_maxdiff(a, b=None):
                                 C.c_void_p(bb.data_ptr() if bb is not None else 0),
                                 C.byref(m), C.byref(d), C.c_void_p(stream)), "tl_mean_maxdiff")
     return m.value, d.value
+
+
+def mma_defaultkws(n: int, n_constr: int, ft=None, device=None):
+    """Default MMA arguments of topoptlab/optimizer/mma_utils.py:8-60 as
+    device tensors (xmin, xmax, low, upp: (n,)) and scalars."""
+    if n_constr == 0:
+        raise ValueError("GCMMA needs to have at least one constraint.")
+    if n_constr != 1:
+        raise NotImplementedError("the device MMA handles one constraint (the volume)")
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else device
+    kw = {"nconstr": n_constr,
+          "xmin": torch.zeros(n, dtype=torch.float64, device=dev),
+          "xmax": torch.ones(n, dtype=torch.float64, device=dev),
+          "low": torch.ones(n, dtype=torch.float64, device=dev),
+          "upp": torch.ones(n, dtype=torch.float64, device=dev),
+          "a0": 1.0, "a": 0.0, "c": 10000.0, "d": 0.0,
+          "move": 0.1 if ft in (5, 6) else (0.05 if ft == 7 else 0.2),
+          "asyinit": 0.5, "asydecr": 0.7, "asyincr": 1.2, "asymin": 0.01, "asymax": 10.0,
+          "raa0": 1e-5, "albefa": 0.1}
+    return kw
+
+
+def mma_update(x, xold1, xold2, df0dx, fval: float, dfdx, kw, iter: int):
+    """One MMA step (the mmapy.mmasub call of topology_optimization.py:591-607
+    for one constraint): returns the new design; kw["low"], kw["upp"] are
+    updated in place, kw["newton_its"] holds the subproblem's Newton steps."""
+    x = torch.as_tensor(x, dtype=torch.float64).reshape(-1).clone()
+    dev = x.device
+    vec = lambda v: None if v is None else \
+        torch.as_tensor(v, dtype=torch.float64, device=dev).reshape(-1).contiguous()
+    xo1, xo2, g0, g1 = vec(xold1), vec(xold2), vec(df0dx), vec(dfdx)
+    opts = _lib.MmaOpts(*(float(kw[k]) for k in ("move", "asyinit", "asydecr", "asyincr",
+                                                 "asymin", "asymax", "raa0", "albefa", "a0", "a",
+                                                 "c", "d")))
+    p = lambda t: C.c_void_p(0 if t is None else t.data_ptr())
+    its = C.c_int32()
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    check(lib().tl_mma_update(x.numel(), p(x), p(xo1), p(xo2), p(kw.get("xmin")),
+                              p(kw.get("xmax")), p(g0), float(fval), p(g1), p(kw["low"]),
+                              p(kw["upp"]), int(iter), C.byref(opts), C.c_void_p(stream),
+                              C.byref(its)), "tl_mma_update")
+    kw["newton_its"] = its.value
+    return x

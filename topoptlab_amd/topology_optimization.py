@@ -25,7 +25,7 @@ from .elements import lk_linear_elast_2d, lk_linear_elast_3d
 from .example_bc import mbb_2d
 from .fem import StiffnessOperator
 from .filter import DeviceFilter, HelmholtzFilter
-from .optimizer import mean_maxdiff, oc_top88
+from .optimizer import mean_maxdiff, mma_defaultkws, mma_update, oc_top88
 
 LOG_LINE = "it.: {0} obj.: {1:.10f} vol.: {2:.10f} ch.: {3:.10f}"
 
@@ -51,7 +51,7 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
          stop_on_change: bool = True, history: Optional[List] = None, device=None):
     """Run the OC loop; return ``(x, obj)`` like topology_optimization.py:688
     (x as a float64 numpy array of shape (nel, 1))."""
-    if optimizer != "oc":
+    if optimizer not in ("oc", "mma"):
         raise NotImplementedError(f"optimizer {optimizer!r} is not on the device path")
     if ft not in (-1, 0, 1):
         raise NotImplementedError(f"filter {ft!r} is not on the device path")
@@ -106,7 +106,16 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
     x = torch.full((n,), float(volfrac), dtype=torch.float64, device=dev)
     xPhys = x.clone()
     g = 0.0
-    dv0 = torch.ones(n, dtype=torch.float64, device=dev)
+    mma = optimizer == "mma"
+    if mma:  # topology_optimization.py:244-246; mma_utils.py:8-60
+        mkw = mma_defaultkws(n, 1, ft, device=dev)
+        mkw.update({k: v for k, v in (optimizer_kw or {}).items() if k not in ("nconstr",)})
+        for k in ("xmin", "xmax", "low", "upp"):
+            mkw[k] = torch.as_tensor(np.asarray(mkw[k].cpu() if torch.is_tensor(mkw[k]) else mkw[k],
+                                                dtype=np.float64).reshape(-1), device=dev).clone()
+        xhist = [x.clone() for _ in range(3)]
+    # constraint gradient (topology_optimization.py:517-520): 1/n for MMA, 1 for OC
+    dv0 = torch.full((n,), 1.0 / n if mma else 1.0, dtype=torch.float64, device=dev)
     if helm:
         dv_f = H.apply_filter(dv0) if ft == 1 else dv0
     else:
@@ -129,7 +138,15 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
         elif ft == 1:
             dc = H.apply(dc, 1)
         xold = x
-        x, g = oc_top88(x, volfrac, dc, dv_f, g)
+        if mma:
+            # mmasub(iter=i) at topology_optimization.py:592-602: i is the
+            # load-case loop's last index (463), 0 for one load case, so the
+            # asymptotes restart every iteration; kept for identical results
+            fval = float(mean_maxdiff(xPhys, None)[0]) - volfrac
+            x = mma_update(x, xhist[-1], xhist[-2], dc, fval, dv_f, mkw, iter=ncases - 1)
+            xhist = (xhist + [x.clone()])[-4:]
+        else:
+            x, g = oc_top88(x, volfrac, dc, dv_f, g)
         if helm:  # 646-647
             xPhys = H.apply_filter(x) if ft == 1 else x.clone()
         else:
