@@ -278,6 +278,41 @@ def bench_helmholtz(dev, nelx=1536, nely=512, rmin=2.4, reps=5):
             "cpu_cores": 1}
 
 
+def bench_mma(dev, n=1_769_472, reps=3):
+    """One MMA design update (volume constraint) on the 192x96x96 design
+    vector with synthetic compliance-like gradients: device (tl_mma_update)
+    against the oracle's mmasub in numpy on the host."""
+    import torch
+    from oracle import mma
+    from topoptlab_amd.optimizer import mma_defaultkws, mma_update
+    rng = np.random.default_rng(0)
+    x = 0.05 + 0.9 * rng.random(n)
+    df0 = -rng.random(n) * 10.0
+    dfdx = np.full(n, 1.0 / n)
+    fval = 0.01
+    t = lambda a: torch.as_tensor(a, device=dev)
+    xd, g0, g1 = t(x), t(df0), t(dfdx)
+    kw = mma_defaultkws(n, 1, device=dev)
+    xm = mma_update(xd, xd, xd, g0, fval, g1, kw, iter=0)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        xm = mma_update(xd, xd, xd, g0, fval, g1, kw, iter=0)
+    torch.cuda.synchronize()
+    ms = 1e3 * (time.perf_counter() - t0) / reps
+    okw = mma.mma_defaultkws(n, 1)
+    t1 = time.perf_counter()
+    out = mma.mmasub(1, n, 0, x[:, None], okw["xmin"], okw["xmax"], x[:, None], x[:, None], 0.0,
+                     df0[:, None], np.array([[fval]]), dfdx[None, :], okw["low"], okw["upp"],
+                     okw["a0"], okw["a"], okw["c"], okw["d"], move=okw["move"])
+    tc = time.perf_counter() - t1
+    err = float(np.linalg.norm(xm.cpu().numpy() - out[0][:, 0]) / np.linalg.norm(out[0]))
+    log(f"mma n={n}: device {ms:.1f} ms per update ({kw['newton_its']} Newton steps), "
+        f"oracle {1e3 * tc:.0f} ms, rel err {err:.1e}")
+    return {"n": n, "ms_per_update": round(ms, 2), "newton_steps": kw["newton_its"],
+            "rel_err_vs_oracle": err, "cpu_oracle_ms": round(1e3 * tc, 1), "cpu_cores": 1}
+
+
 def roofline_entry(kind_def, avg_ms, traffic):
     name, bytes_alg, flops, bound, peak = kind_def
     if bound == "mfma":
@@ -480,9 +515,10 @@ def main():
                           f"scipy sparse kernels run on 1 of {os.cpu_count()} host cores")}
         log(f"cpu baseline sample: {tc:.2f} s for {ndof_s} DOF ({itc} its)")
 
-    helmholtz = None
+    helmholtz = mma_step = None
     if not args.no_variants:
         helmholtz = bench_helmholtz(dev)
+        mma_step = bench_mma(dev)
 
     batched = None
     if not args.no_batched:
@@ -517,6 +553,7 @@ def main():
         "variants": variants,
         "batched": batched,
         "helmholtz_filter": helmholtz,
+        "mma_update": mma_step,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
