@@ -158,6 +158,10 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
             log.info(LOG_LINE.format(loop + 1, obj, vol, change))
         if stop_on_change and change < 0.01:
             break
+    if output_kw.get("export"):  # topology_optimization.py:675-679
+        from .output_designs import export_vtk
+        export_vtk(output_kw["file"], nelx, nely, xPhys.cpu().numpy(), nelz,
+                   x=x.cpu().numpy(), u=u.T.cpu().numpy(), f=f.T.cpu().numpy(), volfrac=volfrac)
     op.close()
     if H is not None:
         H.close()
