@@ -64,104 +64,125 @@ __device__ inline int kidx(int dim, int rx, int ry, int rz) {
 // ---------------------------------------------------------------------------
 // level-1 stencil straight from the fine densities
 // ---------------------------------------------------------------------------
-__global__ void st_from_fine_kernel(GridD g1, const double* __restrict__ spf, int fnelx,
-                                    int fnely, const double* __restrict__ G1, int ndpn,
-                                    const uint8_t* __restrict__ mask1, double* __restrict__ st) {
-  const int ns = nstencil(g1.dim);
-  const int nq = ns * ndpn * ndpn;
-  const int nd = (g1.dim == 3 ? 8 : 4) * ndpn;
-  const int nch = g1.dim == 3 ? 8 : 4;
-  const int64_t tot = g1.nnodes * nq;
+// One thread per (node, d, c): the node's (up to) 8 level-1 elements and
+// their 8 children's scales are loaded once and feed all 27 offsets; each
+// entry sums elements in (oz, ox, oy) order and children in order.
+template <bool D3, int NDPN>
+__global__ __launch_bounds__(BS) void st_from_fine_kernel(GridD g1, const double* __restrict__ spf,
+                                                          int fnelx, int fnely,
+                                                          const double* __restrict__ G1,
+                                                          const uint8_t* __restrict__ mask1,
+                                                          double* __restrict__ st) {
+  constexpr int NEL = D3 ? 8 : 4, NCH = D3 ? 8 : 4, ND = NEL * NDPN, NK = D3 ? 27 : 9;
+  const int64_t tot = g1.nnodes * NDPN * NDPN;
   for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
        t += int64_t(gridDim.x) * BS) {
     const int64_t n = t % g1.nnodes;
-    const int q = int(t / g1.nnodes);
-    const int c = q % ndpn, d = (q / ndpn) % ndpn, k = q / (ndpn * ndpn);
-    int rx, ry, rz;
-    if (g1.dim == 3) { rz = k / 9 - 1; rx = (k / 3) % 3 - 1; ry = k % 3 - 1; }
-    else { rz = 0; rx = k / 3 - 1; ry = k % 3 - 1; }
+    const int dc = int(t / g1.nnodes);
+    const int d = dc / NDPN, c = dc % NDPN;
     int ix, iy, iz;
     node_xyz(g1, n, ix, iy, iz);
-    double acc = 0.0;
-    const bool mexists = node_in(g1, ix + rx, iy + ry, iz + rz);
-    if (mexists) {
-      for (int oz = (g1.dim == 3 ? -1 : 0); oz <= 0; ++oz)
-        for (int ox = -1; ox <= 0; ++ox)
-          for (int oy = -1; oy <= 0; ++oy) {
-            const int ex = ix + ox, ey = iy + oy, ez = iz + oz;
-            if (ex < 0 || ex >= g1.nelx || ey < 0 || ey >= g1.nely) continue;
-            if (g1.dim == 3 && (ez < 0 || ez >= g1.nelz)) continue;
-            const int bx = rx - ox, by = ry - oy, bz = rz - oz;
-            if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
-            const int la = loc_of(-ox, -oy, -oz), lb = loc_of(bx, by, bz);
-            const int ent = (la * ndpn + d) * nd + lb * ndpn + c;
-            for (int ch = 0; ch < nch; ++ch) {
-              const int cy = ch & 1, cx = (ch >> 1) & 1, cz = ch >> 2;
-              const int fx = 2 * ex + cx, fy = 2 * ey + cy, fz = 2 * ez + cz;
-              const int64_t pidx =
-                  (g1.dim == 3 ? (int64_t(fz + 1) * (fnelx + 2) + fx + 1) * (fnely + 2)
-                               : int64_t(fx + 1) * (fnely + 2)) + fy + 1;
-              acc += spf[pidx] * G1[ch * nd * nd + ent];
-            }
-          }
+    // child scales of the elements around the node, e = ((oz+1)*2 + (ox+1))*2 + (oy+1)
+    double sc[NEL][NCH];
+    bool eok[NEL];
+#pragma unroll
+    for (int e = 0; e < NEL; ++e) {
+      const int oz = D3 ? (e >> 2) - 1 : 0, ox = ((e >> 1) & 1) - 1, oy = (e & 1) - 1;
+      const int ex = ix + ox, ey = iy + oy, ez = iz + oz;
+      eok[e] = ex >= 0 && ex < g1.nelx && ey >= 0 && ey < g1.nely &&
+               (!D3 || (ez >= 0 && ez < g1.nelz));
+#pragma unroll
+      for (int ch = 0; ch < NCH; ++ch) {
+        const int cy = ch & 1, cx = (ch >> 1) & 1, cz = ch >> 2;
+        const int fx = 2 * ex + cx, fy = 2 * ey + cy, fz = 2 * ez + cz;
+        const int64_t pidx =
+            (D3 ? (int64_t(fz + 1) * (fnelx + 2) + fx + 1) * (fnely + 2)
+                : int64_t(fx + 1) * (fnely + 2)) + fy + 1;
+        sc[e][ch] = eok[e] ? spf[pidx] : 0.0;
+      }
     }
     const bool fn = (mask1[n] >> d) & 1;
-    const bool fm = mexists ? ((mask1[node_id(g1, ix + rx, iy + ry, iz + rz)] >> c) & 1) : true;
-    if (fn || fm) acc = (fn && rx == 0 && ry == 0 && rz == 0 && c == d) ? 1.0 : 0.0;
-    st[int64_t(q) * g1.nnodes + n] = acc;
+    const double* Gdc = G1 + d * ND + c;  // entry (la*NDPN + d)*ND + lb*NDPN + c
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      const int rz = D3 ? k / 9 - 1 : 0, rx = (k / 3) % 3 - 1, ry = k % 3 - 1;
+      double acc = 0.0;
+      const bool mexists = node_in(g1, ix + rx, iy + ry, iz + rz);
+#pragma unroll
+      for (int e = 0; e < NEL; ++e) {
+        const int oz = D3 ? (e >> 2) - 1 : 0, ox = ((e >> 1) & 1) - 1, oy = (e & 1) - 1;
+        const int bx = rx - ox, by = ry - oy, bz = rz - oz;
+        if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;
+        if (!(mexists && eok[e])) continue;
+        const int la = loc_of(-ox, -oy, -oz), lb = loc_of(bx, by, bz);
+        const int off = la * NDPN * ND + lb * NDPN;
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) acc += sc[e][ch] * Gdc[ch * ND * ND + off];
+      }
+      const bool fm = mexists ? ((mask1[node_id(g1, ix + rx, iy + ry, iz + rz)] >> c) & 1) : true;
+      if (fn || fm) acc = (fn && rx == 0 && ry == 0 && rz == 0 && c == d) ? 1.0 : 0.0;
+      st[int64_t((k * NDPN + d) * NDPN + c) * g1.nnodes + n] = acc;
+    }
   }
 }
 
 // ---------------------------------------------------------------------------
 // stencil Galerkin: level l (fine) -> level l+1 (coarse)
 // ---------------------------------------------------------------------------
-__global__ void st_coarsen_kernel(GridD gf, GridD gc, int ndpn, const double* __restrict__ stf,
-                                  const uint8_t* __restrict__ maskc, double* __restrict__ stc) {
-  const int ns = nstencil(gc.dim);
-  const int nq = ns * ndpn * ndpn;
-  const int64_t tot = gc.nnodes * nq;
-  const bool d3 = gc.dim == 3;
+// One thread per (coarse node, d, c): every fine stencil value of the 27
+// fine nodes around 2n is read once and added to the 1..8 coarse offsets r
+// it feeds (|dx + sx - 2 rx| <= 1 per axis).  Each coarse entry receives its
+// terms in (fine node, fine offset) order, as in the plain triple product.
+template <bool D3>
+__global__ __launch_bounds__(BS) void st_coarsen_kernel(GridD gf, GridD gc, int ndpn,
+                                                        const double* __restrict__ stf,
+                                                        const uint8_t* __restrict__ maskc,
+                                                        double* __restrict__ stc) {
+  constexpr int NK = D3 ? 27 : 9;
+  const int64_t tot = gc.nnodes * ndpn * ndpn;
   for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
        t += int64_t(gridDim.x) * BS) {
     const int64_t n = t % gc.nnodes;
-    const int q = int(t / gc.nnodes);
-    const int c = q % ndpn, d = (q / ndpn) % ndpn, k = q / (ndpn * ndpn);
-    int rx, ry, rz;
-    if (d3) { rz = k / 9 - 1; rx = (k / 3) % 3 - 1; ry = k % 3 - 1; }
-    else { rz = 0; rx = k / 3 - 1; ry = k % 3 - 1; }
+    const int dc = int(t / gc.nnodes);
+    const int d = dc / ndpn, c = dc % ndpn;
     int ix, iy, iz;
     node_xyz(gc, n, ix, iy, iz);
-    double acc = 0.0;
-    const bool mexists = node_in(gc, ix + rx, iy + ry, iz + rz);
-    if (mexists) {
-      for (int dz = (d3 ? -1 : 0); dz <= (d3 ? 1 : 0); ++dz)
-        for (int dx = -1; dx <= 1; ++dx)
-          for (int dy = -1; dy <= 1; ++dy) {
-            const int fx = 2 * ix + dx, fy = 2 * iy + dy, fz = d3 ? 2 * iz + dz : 0;
-            if (!node_in(gf, fx, fy, fz)) continue;
-            const double wi = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
-            const int64_t fi = node_id(gf, fx, fy, fz);
-            for (int sz = (d3 ? -1 : 0); sz <= (d3 ? 1 : 0); ++sz) {
-              const int ez = dz + sz - 2 * rz;
-              if (ez < -1 || ez > 1) continue;
-              for (int sx = -1; sx <= 1; ++sx) {
-                const int ex = dx + sx - 2 * rx;
-                if (ex < -1 || ex > 1) continue;
-                for (int sy = -1; sy <= 1; ++sy) {
-                  const int ey = dy + sy - 2 * ry;
-                  if (ey < -1 || ey > 1) continue;
-                  const double wj = (ex ? 0.5 : 1.0) * (ey ? 0.5 : 1.0) * (ez ? 0.5 : 1.0);
-                  const int ks = kidx(gf.dim, sx, sy, sz);
-                  acc += wi * wj * stf[int64_t((ks * ndpn + d) * ndpn + c) * gf.nnodes + fi];
-                }
-              }
+    double acc[NK];
+#pragma unroll
+    for (int k = 0; k < NK; ++k) acc[k] = 0.0;
+    // fine neighbour, fine offset and coarse offset all unrolled: only the
+    // valid (|dx + sx - 2 rx| <= 1 per axis) multiply-adds are emitted
+#pragma unroll
+    for (int dp = 0; dp < NK; ++dp) {
+          const int dz = D3 ? dp / 9 - 1 : 0, dx = (dp / 3) % 3 - 1, dy = dp % 3 - 1;
+          const int fx = 2 * ix + dx, fy = 2 * iy + dy, fz = D3 ? 2 * iz + dz : 0;
+          if (!node_in(gf, fx, fy, fz)) continue;
+          const double wi = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+          const int64_t fi = node_id(gf, fx, fy, fz);
+#pragma unroll
+          for (int ks = 0; ks < NK; ++ks) {
+            const int sz = D3 ? ks / 9 - 1 : 0, sx = (ks / 3) % 3 - 1, sy = ks % 3 - 1;
+            const double v = stf[int64_t((ks * ndpn + d) * ndpn + c) * gf.nnodes + fi];
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+              const int rz = D3 ? k / 9 - 1 : 0, rx = (k / 3) % 3 - 1, ry = k % 3 - 1;
+              const int ez = dz + sz - 2 * rz, ex = dx + sx - 2 * rx, ey = dy + sy - 2 * ry;
+              if (ez < -1 || ez > 1 || ex < -1 || ex > 1 || ey < -1 || ey > 1) continue;
+              const double wj = (ex ? 0.5 : 1.0) * (ey ? 0.5 : 1.0) * (ez ? 0.5 : 1.0);
+              acc[k] += wi * wj * v;
             }
           }
     }
     const bool fn = (maskc[n] >> d) & 1;
-    const bool fm = mexists ? ((maskc[node_id(gc, ix + rx, iy + ry, iz + rz)] >> c) & 1) : true;
-    if (fn || fm) acc = (fn && rx == 0 && ry == 0 && rz == 0 && c == d) ? 1.0 : 0.0;
-    stc[int64_t(q) * gc.nnodes + n] = acc;
+#pragma unroll
+    for (int k = 0; k < NK; ++k) {
+      const int rz = D3 ? k / 9 - 1 : 0, rx = (k / 3) % 3 - 1, ry = k % 3 - 1;
+      const bool mexists = node_in(gc, ix + rx, iy + ry, iz + rz);
+      double v = mexists ? acc[k] : 0.0;
+      const bool fm = mexists ? ((maskc[node_id(gc, ix + rx, iy + ry, iz + rz)] >> c) & 1) : true;
+      if (fn || fm) v = (fn && rx == 0 && ry == 0 && rz == 0 && c == d) ? 1.0 : 0.0;
+      stc[int64_t((k * ndpn + d) * ndpn + c) * gc.nnodes + n] = v;
+    }
   }
 }
 
@@ -389,15 +410,15 @@ __device__ inline void gj_invert_pivot(double (*a)[GJB + 1]) {
   // in-place Gauss-Jordan inverse of the GJB x GJB LDS block, BS threads
   for (int t = 0; t < GJB; ++t) {
     double nv[GJB * GJB / BS];
-    const double att = a[t][t];
+    const double inv = 1.0 / a[t][t];  // one reciprocal per step
 #pragma unroll
     for (int u = 0; u < GJB * GJB / BS; ++u) {
       const int e = threadIdx.x + u * BS, i = e / GJB, j = e % GJB;
-      const double ait = a[i][t], atj = a[t][j], aij = a[i][j];
-      if (i == t && j == t) nv[u] = 1.0 / att;
-      else if (i == t) nv[u] = atj / att;
-      else if (j == t) nv[u] = -ait / att;
-      else nv[u] = aij - ait * atj / att;
+      const double ait = a[i][t], atj = a[t][j] * inv, aij = a[i][j];
+      if (i == t && j == t) nv[u] = inv;
+      else if (i == t) nv[u] = atj;
+      else if (j == t) nv[u] = -ait * inv;
+      else nv[u] = aij - ait * atj;
     }
     __syncthreads();
 #pragma unroll
@@ -812,18 +833,33 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
   // level 1 from the fine densities
   {
     Level& l1 = M.lv[1];
-    const int64_t tot = l1.g.nnodes * nstencil(g.dim) * ndpn * ndpn;
-    hipLaunchKernelGGL(st_from_fine_kernel, dim3(gsz(tot)), dim3(BS), 0, s, to_d(l1.g),
-                       P->op.sp64.as<double>(), g.nelx, g.nely, M.G1.as<double>(), ndpn,
-                       l1.mask.as<uint8_t>(), l1.st64.as<double>());
+    const int64_t tot = l1.g.nnodes * ndpn * ndpn;
+#define TL_SFF(D3, NDPN)                                                                      \
+  hipLaunchKernelGGL((st_from_fine_kernel<D3, NDPN>), dim3(gsz(tot)), dim3(BS), 0, s,          \
+                     to_d(l1.g), P->op.sp64.as<double>(), g.nelx, g.nely, M.G1.as<double>(),   \
+                     l1.mask.as<uint8_t>(), l1.st64.as<double>())
+    const bool d3 = g.dim == 3;
+    switch (ndpn) {
+      case 1: if (d3) TL_SFF(true, 1); else TL_SFF(false, 1); break;
+      case 2: if (d3) TL_SFF(true, 2); else TL_SFF(false, 2); break;
+      case 3: if (d3) TL_SFF(true, 3); else TL_SFF(false, 3); break;
+      default: set_error("unsupported DOFs per node"); return TL_EINVAL;
+    }
+#undef TL_SFF
     TL_CHECK_LAUNCH();
   }
   for (int l = 2; l < L; ++l) {
     Level& lf = M.lv[l - 1];
     Level& lc = M.lv[l];
-    const int64_t tot = lc.g.nnodes * nstencil(g.dim) * ndpn * ndpn;
-    hipLaunchKernelGGL(st_coarsen_kernel, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lf.g), to_d(lc.g),
-                       ndpn, lf.st64.as<double>(), lc.mask.as<uint8_t>(), lc.st64.as<double>());
+    const int64_t tot = lc.g.nnodes * ndpn * ndpn;
+    if (g.dim == 3)
+      hipLaunchKernelGGL(st_coarsen_kernel<true>, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lf.g),
+                         to_d(lc.g), ndpn, lf.st64.as<double>(), lc.mask.as<uint8_t>(),
+                         lc.st64.as<double>());
+    else
+      hipLaunchKernelGGL(st_coarsen_kernel<false>, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lf.g),
+                         to_d(lc.g), ndpn, lf.st64.as<double>(), lc.mask.as<uint8_t>(),
+                         lc.st64.as<double>());
     TL_CHECK_LAUNCH();
   }
   for (int l = 1; l < L; ++l) {
