@@ -94,7 +94,7 @@ def cpu_baseline(sample=(96, 48, 48), nlevels=4):
     return t, it, ndof
 
 
-def kernel_kinds(dims, vcycle, nlevels):
+def kernel_kinds(dims, vcycle, nlevels, fp64_mfma=False):
     """Launch kinds timed through tl_profile: name, algorithmic bytes and
     flops per launch, roofline bound and peak."""
     nelx, nely, nelz = dims
@@ -108,7 +108,7 @@ def kernel_kinds(dims, vcycle, nlevels):
     if mfma:
         vk = f"kucol_kernel {vcycle.upper()}-input MFMA (fp32 accumulate)"
         vbound, vpeak = "hbm", HBM_PEAK_GBS
-    elif vcycle == "fp64":
+    elif vcycle == "fp64" and fp64_mfma:
         vk = "kucol64_kernel FP64 MFMA (v_mfma_f64_16x16x4f64)"
         vbound, vpeak = "mfma", FP64_PEAK_TFLOPS
     else:
@@ -116,7 +116,8 @@ def kernel_kinds(dims, vcycle, nlevels):
         vbound, vpeak = "mfma", FP32_PEAK_TFLOPS
     sb = 14 * 9 * nn1 * vb  # level-1 block stencil bytes (forward half: centre + 13 blocks)
     return {
-        0: ("kucol64_kernel FP64 MFMA fused PCG product (p = z + beta p; q = K p; p.q)",
+        0: ("kucol64_kernel FP64 MFMA fused PCG product (p = z + beta p; q = K p; p.q)" if fp64_mfma
+            else "ku3d_kernel FP64 fused PCG product (p = z + beta p; q = K p; p.q)",
             32 * ndof + 8 * nel, flops_ku, "mfma", FP64_PEAK_TFLOPS),
         1: (f"{vk} V-cycle residual (r = b - K x)",
             (2 * vb + 8) * ndof + vb * nel + ndof // 3, flops_ku, vbound, vpeak),
@@ -338,6 +339,8 @@ def main():
                     help="V-cycle precision of the timed (headline) solve; fp64 = plain FP64 GMG-PCG")
     ap.add_argument("--gamma", type=int, default=1, help="cycle index (1 V, 2 W) of the timed solve")
     ap.add_argument("--gamma-from", type=int, default=2)
+    ap.add_argument("--fp64-operator", default="vector", choices=["vector", "mfma"],
+                    help="FP64 3D fine operator: MFMA column walk or vector FMAs")
     ap.add_argument("--nelx", type=int, default=192)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true",
@@ -391,6 +394,7 @@ def main():
     x = torch.as_tensor(x_h, device=dev)
     f = torch.as_tensor(f_h[:, 0], device=dev)
     lib = _lib.lib()
+    _lib.check(lib.tl_set_fp64_operator(op._h, 1 if args.fp64_operator == "mfma" else 0))
 
     def step(vcycle, gamma, gamma_from, rtol=1e-8):
         op.set_density(x)
@@ -446,7 +450,7 @@ def main():
     compliance = float(torch.dot(f, u).item())
     log(f"solve ({args.vcycle} V-cycle, gamma {args.gamma}): {ms:.2f} ms, {iters} PCG iterations, "
         f"true relres {relres:.3e}, compliance {compliance:.10e}")
-    kinds = kernel_kinds(dims, args.vcycle, args.levels)
+    kinds = kernel_kinds(dims, args.vcycle, args.levels, args.fp64_operator == "mfma")
     for k, (tot, cnt) in prof.items():
         if cnt:
             avg = tot / cnt
@@ -462,7 +466,8 @@ def main():
     hbm = None
     tfile = ROOT / "profiles" / "traffic.json"
     if tfile.exists():
-        hbm = json.loads(tfile.read_text()).get(f"kind{dom}_{args.vcycle}_{nelx}")
+        op_key = "_mfma" if (args.vcycle == "fp64" and args.fp64_operator == "mfma") else ""
+        hbm = json.loads(tfile.read_text()).get(f"kind{dom}_{args.vcycle}{op_key}_{nelx}")
     roof = roofline_entry(kinds[dom], avg_ms, hbm)
     log(f"dominant kernel: {roof['kernel']}: {roof['avg_us']} us/launch, "
         f"{roof['achieved']} {roof['unit']} ({100 * roof['frac']:.1f}% of peak)")

@@ -123,9 +123,9 @@ int tl_batch_solve(int32_t n, tl_problem* const* handles, const double* const* f
                    double* const* u, const int32_t* nrhs, const tl_solve_opts* opts,
                    const tl_batch_opts* bopts, tl_solve_info* infos);
 
-/* FP64 fine-grid operator of 3D H8 elasticity: kind 1 (the default) runs
- * the element contraction on v_mfma_f64_16x16x4f64 (column walk), kind 0 on
- * vector FMAs; same operator, different summation order (no counterpart in
+/* FP64 fine-grid operator of 3D H8 elasticity: kind 0 (the default) runs
+ * the element contraction on vector FMAs, kind 1 on v_mfma_f64_16x16x4f64
+ * (column walk); same operator, different summation order (no counterpart in
  * topoptlab: the choice of kernel for K.u of fem.py:141-178). */
 int tl_set_fp64_operator(tl_problem* p, int32_t kind);
 

@@ -363,8 +363,8 @@ def test_mfma_vcycle_pcg_converges(gpu, dtype):
 
 @pytest.mark.parametrize("dims", [(6, 4, 5), (24, 16, 20), (17, 30, 9)])
 def test_fp64_mfma_operator_matches_vector(gpu, dims):
-    """The FP64 column walk on v_mfma_f64_16x16x4f64 (default for 3D H8
-    elasticity) against the vector-FMA kernel and the assembled oracle: K.u
+    """The FP64 column walk on v_mfma_f64_16x16x4f64 (selectable for 3D H8
+    elasticity) against the vector-FMA kernel (the default) and the assembled oracle: K.u
     to 1e-13, and GMG-PCG with either operator in every FP64 mode (fused
     product, V-cycle residual and Jacobi sweeps) to the same iteration count."""
     import torch

@@ -285,16 +285,21 @@ __device__ inline void load_dens(const KuArgs& a, int ix, int iy, int ez, T (&S)
 }
 
 // Register prefetch of one staged plane: slot k holds LDS entry
-// threadIdx.x + k*BS.  PF covers planes with (BS + 2*ny + 2)*NDPN <= PF*BS
-// (ny <= 169 for NDPN = 3); entries beyond are staged synchronously.
-template <int NDPN>
+// threadIdx.x + k*BS.  PF slots cover planes with (BS + 2*ny + 2)*NDPN <=
+// PF*BS; entries beyond are staged synchronously.  FP64 with 3 DOFs per node
+// keeps 4 slots: with 7 (every plane up to ny = 169) the kernel needed 256
+// VGPRs plus AGPR spills and ran one wave per SIMD; with 4 it fits two waves
+// per SIMD, and the second wave hides the synchronous part (measured on the
+// 192x96x96 grid: 128 -> 102 us per fused product; 2, 3 or 0 slots are slower).
+template <int NDPN, typename T = float>
 struct PfCount {
-  static constexpr int value = NDPN == 3 ? 7 : (NDPN == 2 ? 5 : 3);
+  static constexpr int value =
+      NDPN == 3 ? (sizeof(T) == 8 ? 4 : 7) : (NDPN == 2 ? 5 : 3);
 };
 
 template <int NDPN, typename T, bool FUSEP>
 struct PlanePrefetch {
-  static constexpr int PF = PfCount<NDPN>::value;
+  static constexpr int PF = PfCount<NDPN, T>::value;
   T a[PF];
   T b[FUSEP ? PF : 1];
 };
@@ -302,7 +307,7 @@ struct PlanePrefetch {
 template <int NDPN, typename T, bool FUSEP>
 __device__ inline void fetch_plane(const KuArgs& a, PlanePrefetch<NDPN, T, FUSEP>& pf, int pz,
                                    int64_t j0) {
-  constexpr int PF = PfCount<NDPN>::value;
+  constexpr int PF = PfCount<NDPN, T>::value;
   const int L = BS + 2 * a.ny + 2;
   const int64_t base = j0 - a.ny - 1;
   const bool zin = pz >= 0 && pz < a.nz;
@@ -324,7 +329,7 @@ __device__ inline void fetch_plane(const KuArgs& a, PlanePrefetch<NDPN, T, FUSEP
 template <int NDPN, typename T, bool FUSEP>
 __device__ inline void store_plane(const KuArgs& a, const PlanePrefetch<NDPN, T, FUSEP>& pf,
                                    T* lds, int pz, int64_t j0) {
-  constexpr int PF = PfCount<NDPN>::value;
+  constexpr int PF = PfCount<NDPN, T>::value;
   const int L = BS + 2 * a.ny + 2;
   T beta = T(0);
   if constexpr (FUSEP) beta = T(*a.beta);
@@ -1152,7 +1157,7 @@ int tl_create(const tl_grid_desc* d, tl_problem** out) {
   }
   P->fixed_h.assign(d->fixed, d->fixed + d->nfixed);
   detect_ke_pattern(P);
-  P->op.mfma64 = P->g.dim == 3 && P->ndpn == 3;
+  P->op.mfma64 = false;  // the vector kernel is faster (tl_set_fp64_operator selects)
   std::vector<float> ke32(nd * nd);
   for (int i = 0; i < nd * nd; ++i) ke32[i] = float(P->KEh[i]);
   auto fail = [&](int rc) { delete P; return rc; };
