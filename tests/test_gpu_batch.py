@@ -156,3 +156,28 @@ def test_batch_matches_single_solves(gpu):
             assert rel(u, spsolve(K, b)) < 1e-4, (i, k)
     for op in ops:
         op.close()
+
+
+def test_batch_reports_non_convergence(gpu):
+    """maxiter too small: TL_ENOCONV from the batch, every system's status
+    says so, and raise_on_fail=False returns the partial results."""
+    import torch
+    from topoptlab_amd.batch import batch_solve, build_instance
+    from topoptlab_amd.workloads import batched_instances
+    specs = batched_instances(seed=4, n_elast=2, n_heat=1, xrange=(64, 96), yrange=(32, 64),
+                              ncases=2)
+    ops, fs = [], []
+    for spec in specs:
+        op, x, f, levels = build_instance(spec, gpu)
+        op.set_density(x)
+        op.setup_mg(nlevels=levels, precision="fp32")
+        ops.append(op)
+        fs.append(torch.as_tensor(f, device=gpu))
+    with pytest.raises(RuntimeError):
+        batch_solve(ops, fs, rtol=1e-12, maxiter=2, chunk=2)
+    U, infos = batch_solve(ops, fs, rtol=1e-12, maxiter=2, chunk=2, raise_on_fail=False)
+    for lst in infos:
+        for inf in lst:
+            assert inf["status"] != 0 and inf["iterations"] <= 4
+    for op in ops:
+        op.close()
