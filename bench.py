@@ -131,7 +131,7 @@ def kernel_kinds(dims, vcycle, nlevels, fp64_mfma=False):
     }
 
 
-def bench_batched(dev, steps=2, nstreams=8, chunk=8, cpu_sample=True):
+def bench_batched(dev, steps=2, nstreams=12, chunk=8, cpu_sample=True):
     """Batched uneven 2D solves: 32 Q4 elasticity grids (4 load cases each)
     and 8 Q4 heat grids from workloads.batched_instances(seed=0), FP64 outer
     PCG with FP32 V-cycles to relres 1e-8.  A step = set_density +

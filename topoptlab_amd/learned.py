@@ -324,6 +324,42 @@ def generate_pairs(nelx, nely, nelz, runs, iters, seed=0, volfrac=0.3, rmin=1.5,
     return X, T
 
 
+def generate_field_pairs(nelx, nely, nelz, count, seed=0, device="cuda", log=None):
+    """(features, target) pairs on random designs: seeded Gaussian random
+    density fields (length scale 2-6 elements, mean 0.2-0.5, in [0.001, 1])
+    under the clamped line load of the 3D cantilever benchmark or a seeded
+    point load, solved by plain GMG-PCG (relres 1e-6)."""
+    from .elements import lk_linear_elast_3d
+    from .example_bc import clamped_lineload_3d
+    from .fem import StiffnessOperator
+    from .workloads import gaussian_random_density
+    rng = np.random.default_rng(seed)
+    KE = lk_linear_elast_3d()
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    X, T = [], []
+    _, f_line, fixed_line, _, _ = clamped_lineload_3d(nelx, nely, nelz, ndof)
+    for k in range(count):
+        x_h = gaussian_random_density(nelx, nely, nelz, seed=int(rng.integers(1 << 30)),
+                                      sigma=float(rng.uniform(2.0, 6.0)),
+                                      mean=float(rng.uniform(0.2, 0.5)))
+        if k % 2 == 0:
+            f_h, fixed = f_line[:, 0], fixed_line
+        else:
+            f_h, fixed = cantilever_point_load(nelx, nely, nelz, rng)
+        op = StiffnessOperator(nelx, nely, nelz, KE=KE, fixed=fixed, device=device)
+        xPhys = torch.as_tensor(x_h, device=device)
+        f = torch.as_tensor(np.ascontiguousarray(f_h), device=device)
+        op.set_density(xPhys)
+        op.setup_mg(precision="fp32")
+        u, info = op.solve(f, rtol=1e-6)
+        X.append(features(op, xPhys, f))
+        T.append(target(op, u))
+        op.close()
+        if log and (k + 1) % 50 == 0:
+            log(f"field pairs: {k + 1} of {count}")
+    return X, T
+
+
 def train(net: CoarseNet, X: List[torch.Tensor], T: List[torch.Tensor], epochs=20, lr=1e-3,
           seed=0, log=None) -> List[float]:
     """Adam over the pairs in seeded random order; returns the mean loss per epoch."""
