@@ -127,7 +127,7 @@ struct KuArgs {
   PcgState* st;
   int fin;
   const int* done;
-  int nelx, nely, nelz, nx, ny, nz, plane, zchunk;
+  int nelx, nely, nelz, nx, ny, nz, plane, ncol;  // ncol: 3D tile columns
   // group values of a pattern-matched element matrix (see ke_pattern.hpp)
   double kv64[8];
   float kv32[8];
@@ -284,71 +284,94 @@ __device__ inline void load_dens(const KuArgs& a, int ix, int iy, int ez, T (&S)
       S[(ox + 1) * 2 + (oy + 1)] = sp[(rowz + ix + ox + 1) * (a.nely + 2) + iy + oy + 1];
 }
 
-// Register prefetch of one staged plane: slot k holds LDS entry
-// threadIdx.x + k*BS.  PF slots cover planes with (BS + 2*ny + 2)*NDPN <=
-// PF*BS; entries beyond are staged synchronously.  FP64 with 3 DOFs per node
-// keeps 4 slots: with 7 (every plane up to ny = 169) the kernel needed 256
-// VGPRs plus AGPR spills and ran one wave per SIMD; with 4 it fits two waves
-// per SIMD, and the second wave hides the synchronous part (measured on the
-// 192x96x96 grid: 128 -> 102 us per fused product; 2, 3 or 0 slots are slower).
-template <int NDPN, typename T = float>
-struct PfCount {
-  static constexpr int value =
-      NDPN == 3 ? (sizeof(T) == 8 ? 4 : 7) : (NDPN == 2 ? 5 : 3);
+// Plane staging of the 3D operator by LDS DMA (global_load_lds): the
+// L*NDPN entries of node plane pz around tile j0 (nodes j0-ny-1 ..
+// j0+BS+ny, L = BS+2ny+2) are copied from HBM straight into LDS, one dword
+// per lane per copy, lane k of a copy writing dword k after the wave's base
+// (the 12-byte form writes 16-byte lane slots, which would leave gaps in the
+// dense plane).  No registers hold the plane in flight, so the copy of plane iz+2
+// overlaps the FMAs of step iz in full (a register prefetch had to stop at
+// 4 slots per thread to keep two waves per SIMD and staged the rest of each
+// plane synchronously).  The compiler waits for the copies (vmcnt) before
+// the next barrier.  Entries outside the array are clamped to its first or
+// last unit: halo nodes outside the plane or the grid only meet elements of
+// the zero-scaled padded border, so any finite value gives the same sums.
+template <int NDPN, typename T>
+struct PlaneDma {
+  static constexpr int UB = 4;  // bytes per lane per copy
+  // copies per plane and LDS bytes per plane buffer (padded to whole rounds)
+  __host__ __device__ static int units(int ny) {
+    return int((int64_t(BS + 2 * ny + 2) * NDPN * sizeof(T) + UB - 1) / UB);
+  }
+  __host__ __device__ static int rounds(int ny) { return (units(ny) + BS - 1) / BS; }
+  __host__ __device__ static size_t buf_bytes(int ny) { return size_t(rounds(ny)) * BS * UB; }
 };
 
-template <int NDPN, typename T, bool FUSEP>
-struct PlanePrefetch {
-  static constexpr int PF = PfCount<NDPN, T>::value;
-  T a[PF];
-  T b[FUSEP ? PF : 1];
-};
-
-template <int NDPN, typename T, bool FUSEP>
-__device__ inline void fetch_plane(const KuArgs& a, PlanePrefetch<NDPN, T, FUSEP>& pf, int pz,
-                                   int64_t j0) {
-  constexpr int PF = PfCount<NDPN, T>::value;
-  const int L = BS + 2 * a.ny + 2;
-  const int64_t base = j0 - a.ny - 1;
-  const bool zin = pz >= 0 && pz < a.nz;
-#pragma unroll
-  for (int k = 0; k < PF; ++k) {
-    const int t = threadIdx.x + k * BS;
-    const int64_t jj = base + t / NDPN;
-    T va = T(0), vb = T(0);
-    if (zin && t < L * NDPN && jj >= 0 && jj < a.plane) {
-      const int64_t g = (int64_t(pz) * a.plane + jj) * NDPN + (t % NDPN);
-      va = ldval<T>(a.u, g);
-      if constexpr (FUSEP) vb = ldval<T>(a.pin, g);
+template <int NDPN, typename T>
+__device__ inline void dma_plane(const KuArgs& a, const void* src, T* lds, int pz, int64_t j0) {
+  constexpr int UB = PlaneDma<NDPN, T>::UB;
+  const int nu = PlaneDma<NDPN, T>::units(a.ny);
+  const int64_t total = int64_t(a.plane) * a.nz * NDPN * int64_t(sizeof(T));
+  const int64_t base = (int64_t(pz) * a.plane + j0 - a.ny - 1) * NDPN * int64_t(sizeof(T));
+  const unsigned wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const char* g = static_cast<const char*>(src);
+  char* l8 = reinterpret_cast<char*>(lds) + wave * 64 * UB;
+  if (base >= 0 && base + int64_t(nu) * UB <= total) {
+    // the whole plane range lies inside the array (every tile but the
+    // first and last of the grid): no clamping
+    const char* gp = g + base + int64_t(threadIdx.x) * UB;
+    for (int r = 0; r * BS < nu; ++r)
+      if (r * BS + int(threadIdx.x) < nu)
+        __builtin_amdgcn_global_load_lds(gp + r * BS * UB, l8 + r * BS * UB, 4, 0, 0);
+    return;
+  }
+  for (int r = 0; r * BS < nu; ++r) {
+    const int k = r * BS + int(threadIdx.x);
+    if (k < nu) {
+      int64_t o = base + int64_t(k) * UB;
+      o = o < 0 ? int64_t(0) : (o > total - UB ? total - UB : o);
+      __builtin_amdgcn_global_load_lds(g + o, l8 + r * BS * UB, 4, 0, 0);
     }
-    pf.a[k] = va;
-    if constexpr (FUSEP) pf.b[k] = vb;
   }
 }
 
-template <int NDPN, typename T, bool FUSEP>
-__device__ inline void store_plane(const KuArgs& a, const PlanePrefetch<NDPN, T, FUSEP>& pf,
-                                   T* lds, int pz, int64_t j0) {
-  constexpr int PF = PfCount<NDPN, T>::value;
-  const int L = BS + 2 * a.ny + 2;
-  T beta = T(0);
-  if constexpr (FUSEP) beta = T(*a.beta);
-#pragma unroll
-  for (int k = 0; k < PF; ++k) {
-    const int t = threadIdx.x + k * BS;
-    if (t < L * NDPN) lds[t] = FUSEP ? pf.a[k] + beta * pf.b[k] : pf.a[k];
-  }
-  // entries past the prefetch slots (very wide planes): staged directly
-  const int64_t base = j0 - a.ny - 1;
-  const bool zin = pz >= 0 && pz < a.nz;
-  for (int t = threadIdx.x + PF * BS; t < L * NDPN; t += BS) {
-    const int64_t jj = base + t / NDPN;
-    T v = T(0);
-    if (zin && jj >= 0 && jj < a.plane)
-      v = stage_val<T>(a, (int64_t(pz) * a.plane + jj) * NDPN + (t % NDPN), FUSEP);
-    lds[t] = v;
+// Wait until this wave's copies (and every other vector memory operation it
+// issued) have completed: vmcnt(0) in the gfx9 s_waitcnt encoding, issued
+// through the builtin so that the compiler's own wait tracking knows that no
+// LDS copy is pending afterwards.
+__device__ inline void wait_copies() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// FUSEP: form the new search direction z + beta*p_old in place on the
+// entries that this wave copied (dwords [r*BS + 64*wave, +64) of every
+// round r: whole entries); call after wait_copies().  The barrier that
+// precedes the plane's first read publishes the result, so no extra
+// barrier is needed.
+template <int NDPN, typename T>
+__device__ inline void combine_own(const KuArgs& a, T* lz, const T* lp, T beta) {
+  constexpr int EPW = 64 * PlaneDma<NDPN, T>::UB / int(sizeof(T));  // entries per wave copy
+  const int n = (BS + 2 * a.ny + 2) * NDPN;
+  const int nu = PlaneDma<NDPN, T>::units(a.ny);
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  if (lane < EPW) {
+    for (int r = 0; r * BS < nu; ++r) {
+      const int e = (r * BS + wave * 64) * PlaneDma<NDPN, T>::UB / int(sizeof(T)) + lane;
+      if (e < n) lz[e] = lz[e] + beta * lp[e];
+    }
   }
 }
+
+// LDS plane buffers of one workgroup: z(parity) for the operator's input
+// (with FUSEP: z, overwritten by the new direction z + beta*p_old) and, with
+// FUSEP, p(parity) for the previous search direction.  Addresses are formed
+// arithmetically (a runtime-indexed pointer array would live in scratch).
+template <typename T>
+struct PlaneBufs {
+  char* base;
+  unsigned bb;  // bytes per buffer
+  __device__ T* z(int parity) const { return reinterpret_cast<T*>(base + parity * bb); }
+  __device__ T* p(int parity) const { return reinterpret_cast<T*>(base + (2 + parity) * bb); }
+};
 
 // Epilogue inputs of one node, fetched at the start of its step.
 template <int NDPN, typename T, int MODE, typename TB>
@@ -450,22 +473,23 @@ __device__ inline void layer_sum(const KuArgs& a, const T (&U)[2][9][NDPN], cons
 }
 
 // One step over element layer iz (node planes iz, iz+1).  On entry: U[SA]
-// holds plane iz, the LDS buffer of plane iz+1 is written (barrier pending),
+// holds plane iz, the copy of plane iz+1 into its LDS buffer is in flight,
 // S holds layer iz's densities and `carry` the layer-(iz-1) part of node iz.
-// The step prefetches plane iz+2 and layer iz+1 into registers, completes
-// and writes node iz, carries node iz+1's partial sum, then stores plane
-// iz+2 into the other LDS buffer.
+// The step reads plane iz+1, starts the copy of plane iz+2 (when a later
+// step needs it) into the buffer plane iz came from, prefetches layer iz+1's
+// densities, completes and writes node iz and carries node iz+1's partial sum.
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST,
           int SA, int SB>
-__device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t j0,
-                              int64_t j, int ix, int iy, bool active, T (&U)[2][9][NDPN],
-                              T (&S)[4], T (&carry)[NDPN], double& dot) {
-  T* cur = ((iz + 1) & 1) ? lds1 : lds0;  // holds plane iz+1
-  T* nxt = ((iz + 2) & 1) ? lds1 : lds0;  // receives plane iz+2
+__device__ inline void z_step(const KuArgs& a, const PlaneBufs<T>& pb, int iz, int z1,
+                              int64_t j0, int64_t j, int ix, int iy, bool active,
+                              T (&U)[2][9][NDPN], T (&S)[4], T (&carry)[NDPN], double& dot) {
+  const int pc = (iz + 1) & 1;  // buffer parity of plane iz+1 (plane iz+2: the other)
   __syncthreads();
-  read_nbrs<NDPN, T, SB>(a, cur, U);
-  PlanePrefetch<NDPN, T, FUSEP> pf;
-  fetch_plane<NDPN, T, FUSEP>(a, pf, iz + 2, j0);
+  read_nbrs<NDPN, T, SB>(a, pb.z(pc), U);
+  if (iz + 1 < z1) {
+    dma_plane<NDPN, T>(a, a.u, pb.z(pc ^ 1), iz + 2, j0);
+    if constexpr (FUSEP) dma_plane<NDPN, T>(a, a.pin, pb.p(pc ^ 1), iz + 2, j0);
+  }
   const int64_t node = int64_t(iz) * a.plane + (active ? j : 0);
   EpiPrefetch<NDPN, T, MODE, TB> ep;
   fetch_epi<NDPN, T, MODE, TB>(a, node, ep);
@@ -476,6 +500,16 @@ __device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t
   load_dens<T>(a, ix, iy, min(iz + 1, a.nelz), Sn);
   T acc_top[NDPN], acc_bot[NDPN];
   layer_sum<NDPN, T, SA, SB, FAST, true, true>(a, U, S, acc_top, acc_bot);
+  // Wait for the copies of plane iz+2 and the epilogue inputs here, after
+  // the FMAs (the empty asm consumes the sums, so the FMAs cannot be sunk
+  // below the wait) and before the stores: no copy is then pending at the
+  // next barrier, and the stores drain across it.
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) asm volatile("" ::"v"(acc_top[d]), "v"(acc_bot[d]));
+  wait_copies();
+  if constexpr (FUSEP) {
+    if (iz + 1 < z1) combine_own<NDPN, T>(a, pb.z(pc ^ 1), pb.p(pc ^ 1), T(*a.beta));
+  }
   if (active) {
     T uc[NDPN], tot[NDPN];
 #pragma unroll
@@ -493,58 +527,98 @@ __device__ inline void z_step(const KuArgs& a, T* lds0, T* lds1, int iz, int64_t
   for (int d = 0; d < NDPN; ++d) carry[d] = acc_bot[d];
 #pragma unroll
   for (int k = 0; k < 4; ++k) S[k] = Sn[k];
-  store_plane<NDPN, T, FUSEP>(a, pf, nxt, iz + 2, j0);
 }
 
+// Node planes z0..z1-1 of the column of tile bx (BS consecutive nodes of a
+// plane, starting at node bx*BS).  Called with the LDS free (all threads
+// past their last read of the previous segment, no copy in flight).
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
-__global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
-  if (a.done && *a.done) return;
-  extern __shared__ double lds_raw[];
-  const int L = BS + 2 * a.ny + 2;
-  T* lds0 = reinterpret_cast<T*>(lds_raw);
-  T* lds1 = lds0 + L * NDPN;
-  // XCD-aware tile order: workgroups are dispatched round-robin over the 8
-  // XCDs, each with its own L2; remap so that XCD x runs a contiguous range
-  // of tiles (plane chunks fastest) and the halos shared by neighbouring
-  // tiles are re-read from its own L2 instead of HBM.
-  const unsigned nb = gridDim.x * gridDim.y;
-  const unsigned hw = blockIdx.y * gridDim.x + blockIdx.x;
-  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
-  const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
-  const unsigned bx = logical % gridDim.x, by = logical / gridDim.x;
+__device__ inline void ku3d_segment(const KuArgs& a, const PlaneBufs<T>& pb, unsigned bx, int z0,
+                                    int z1, double& dot) {
   const int64_t j0 = int64_t(bx) * BS;
   const int64_t j = j0 + threadIdx.x;
   const bool inplane = j < a.plane;
   const int64_t jc = inplane ? j : a.plane - 1;
   const int ix = int(jc / a.ny), iy = int(jc % a.ny);
-  const int z0 = by * a.zchunk;
-  const int z1 = min(a.nz, z0 + a.zchunk);
   T U[2][9][NDPN];
   T S[4];
   T carry[NDPN];
-  double dot = 0.0;
   // prologue: planes z0-1 -> U[0], z0 -> U[1]; layer z0-1 gives node z0's
-  // "below" part; plane z0+1 staged for the first step; layer z0 densities
-  T* b_m = ((z0 + 1) & 1) ? lds1 : lds0;  // parity of z0-1 == parity of z0+1
-  T* b_c = (z0 & 1) ? lds1 : lds0;
-  stage_plane<NDPN, T>(a, b_m, z0 - 1, j0, FUSEP);
-  stage_plane<NDPN, T>(a, b_c, z0, j0, FUSEP);
+  // "below" part; the copy of plane z0+1 starts for the first step; layer
+  // z0 densities
+  const int pm = (z0 + 1) & 1;  // parity of z0-1 == parity of z0+1
+  dma_plane<NDPN, T>(a, a.u, pb.z(pm), z0 - 1, j0);
+  dma_plane<NDPN, T>(a, a.u, pb.z(pm ^ 1), z0, j0);
+  if constexpr (FUSEP) {
+    dma_plane<NDPN, T>(a, a.pin, pb.p(pm), z0 - 1, j0);
+    dma_plane<NDPN, T>(a, a.pin, pb.p(pm ^ 1), z0, j0);
+  }
   load_dens<T>(a, ix, iy, z0 - 1, S);
+  wait_copies();
+  if constexpr (FUSEP) {
+    const T beta = T(*a.beta);
+    combine_own<NDPN, T>(a, pb.z(pm), pb.p(pm), beta);
+    combine_own<NDPN, T>(a, pb.z(pm ^ 1), pb.p(pm ^ 1), beta);
+  }
   __syncthreads();
-  read_nbrs<NDPN, T, 0>(a, b_m, U);
-  read_nbrs<NDPN, T, 1>(a, b_c, U);
+  read_nbrs<NDPN, T, 0>(a, pb.z(pm), U);
+  read_nbrs<NDPN, T, 1>(a, pb.z(pm ^ 1), U);
   {
     T unused[NDPN];
     layer_sum<NDPN, T, 0, 1, FAST, false, true>(a, U, S, unused, carry);
   }
-  __syncthreads();
-  stage_plane<NDPN, T>(a, b_m, z0 + 1, j0, FUSEP);
+  __syncthreads();  // every wave has read plane z0-1 before its buffer is refilled
+  dma_plane<NDPN, T>(a, a.u, pb.z(pm), z0 + 1, j0);
+  if constexpr (FUSEP) dma_plane<NDPN, T>(a, a.pin, pb.p(pm), z0 + 1, j0);
   load_dens<T>(a, ix, iy, z0, S);
+  // no copy may be pending where the step loop starts: the compiler would
+  // otherwise wait for every memory operation at the loop's barrier, the
+  // previous step's stores included
+  wait_copies();
+  if constexpr (FUSEP) combine_own<NDPN, T>(a, pb.z(pm), pb.p(pm), T(*a.beta));
   // alternating U slots: plane iz in slot SA, plane iz+1 read into SB
   for (int iz = z0; iz < z1; iz += 2) {
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 1, 0>(a, lds0, lds1, iz, j0, j, ix, iy, inplane, U, S, carry, dot);
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 1, 0>(a, pb, iz, z1, j0, j, ix, iy, inplane, U, S, carry, dot);
     if (iz + 1 >= z1) break;
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 0, 1>(a, lds0, lds1, iz + 1, j0, j, ix, iy, inplane, U, S, carry, dot);
+    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 0, 1>(a, pb, iz + 1, z1, j0, j, ix, iy, inplane, U, S, carry, dot);
+  }
+}
+
+// One workgroup per resident slot (the grid is sized by the launcher from
+// the kernel's occupancy); the ncol*nz (tile column, node plane) units are
+// split evenly over the grid in column-major order, so every workgroup
+// walks about the same number of planes, in one or two column segments.
+// A grid larger than the resident slots would leave a partial last round.
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
+__global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
+  if (a.done && *a.done) return;
+  extern __shared__ double lds_raw[];
+  PlaneBufs<T> pb;
+  pb.base = reinterpret_cast<char*>(lds_raw);
+  pb.bb = unsigned(PlaneDma<NDPN, T>::buf_bytes(a.ny));
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
+  // each with its own L2; remap so that XCD x runs a contiguous range of
+  // units and the halos shared by neighbouring tiles are re-read from its
+  // own L2 instead of HBM.
+  const unsigned nb = gridDim.x;
+  const unsigned hw = blockIdx.x;
+  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned w = xcd * q + min(xcd, r) + (hw >> 3);
+  // unit range [u0, u1) of this workgroup (32-bit: the launcher checks that
+  // units * gridDim.x fits); wave-uniform values kept in scalar registers
+  const unsigned units = unsigned(a.ncol) * unsigned(a.nz);
+  const unsigned u0 = __builtin_amdgcn_readfirstlane(units * w / nb);
+  const unsigned u1 = __builtin_amdgcn_readfirstlane(units * (w + 1) / nb);
+  double dot = 0.0;
+  // the range spans at most two columns (the launcher keeps it <= nz units)
+  const unsigned bx = __builtin_amdgcn_readfirstlane(u0 / unsigned(a.nz));
+  const int z0 = int(u0 - bx * unsigned(a.nz));
+  const int z1 = min(a.nz, z0 + int(u1 - u0));
+  ku3d_segment<NDPN, T, MODE, TB, TO, FUSEP, FAST>(a, pb, bx, z0, z1, dot);
+  const int rest = int(u1 - u0) - (z1 - z0);
+  if (rest > 0) {
+    __syncthreads();  // the next segment's prologue reuses both LDS buffers
+    ku3d_segment<NDPN, T, MODE, TB, TO, FUSEP, FAST>(a, pb, bx + 1, 0, rest, dot);
   }
   if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
     __shared__ double sh[BS / 64 + 2];
@@ -669,6 +743,32 @@ __global__ __launch_bounds__(BS) void ku2t_kernel(KuArgs a) {
   }
 }
 
+// Workgroups of BS threads that the current device holds at once for
+// `kernel` with `lds` bytes of dynamic LDS: CUs x resident workgroups per CU
+// (cached per device, kernel and LDS size).
+static int resident_slots(const void* kernel, size_t lds, int64_t& slots) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, const void*, size_t>, int64_t> cache;
+  int dev = 0;
+  TL_HIP(hipGetDevice(&dev));
+  const auto key = std::make_tuple(dev, kernel, lds);
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+      slots = it->second;
+      return TL_OK;
+    }
+  }
+  int ncu = 0, per = 0;
+  TL_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  TL_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, BS, lds));
+  slots = int64_t(std::max(ncu, 1)) * std::max(per, 1);
+  std::lock_guard<std::mutex> lk(mu);
+  cache[key] = slots;
+  return TL_OK;
+}
+
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
 static int launch_ku2t(const tl_problem* P, const KuArgs& a) {
   constexpr int TX = BS / TY;
@@ -694,25 +794,34 @@ static int launch_ku_impl(const tl_problem* P, KuArgs a) {
   const int L = BS + 2 * g.ny + 2;
   const unsigned nbx = unsigned((int64_t(a.plane) + BS - 1) / BS);
   if (g.dim == 3) {
-    // z chunk: enough workgroups to fill 256 CUs, each chunk >= 4 planes
-    // z chunk: ~1024 workgroups (4 per CU), chunks of >= 6 planes so the
-    // z halo (2 extra planes per chunk) stays small
-    int zc = g.nz;
-    while (zc > 6 && int64_t(nbx) * ((g.nz + zc - 1) / zc) < 1024) zc = (zc + 1) / 2;
-    a.zchunk = zc;
-    dim3 grid(nbx, unsigned((g.nz + zc - 1) / zc));
-    if (int64_t(grid.x) * grid.y > kMaxRedBlocks && (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)) {
+    // one workgroup per resident slot, at least 4 planes each
+    const size_t lds = (FUSEP ? 4 : 2) * PlaneDma<NDPN, T>::buf_bytes(g.ny);
+    TL_ARG(lds <= size_t(160) * 1024, "grid too wide in y for the 3D operator's LDS planes");
+    const int64_t units = int64_t(nbx) * g.nz;
+    int64_t slots = 0;
+    TL_TRY(resident_slots(reinterpret_cast<const void*>(
+                              ku3d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>),
+                          lds, slots));
+    // at least one workgroup per column: a range of <= nz units spans at
+    // most two columns, which is what the kernel walks
+    const int64_t nwg =
+        std::max<int64_t>(nbx, std::min<int64_t>(slots, (units + 3) / 4));
+    // the kernel's unit arithmetic is 32-bit
+    TL_ARG(units * (nwg + 1) < (int64_t(1) << 32), "grid too large for the 3D operator");
+    if (nwg > kMaxRedBlocks && (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)) {
       set_error("fine operator grid exceeds the reduction slot count");
       return TL_EINVAL;
     }
-    hipLaunchKernelGGL((ku3d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), grid, dim3(BS),
-                       size_t(2) * L * NDPN * sizeof(T), P->stream, a);
+    a.ncol = int(nbx);
+    const dim3 grid(static_cast<unsigned>(nwg));
+    hipLaunchKernelGGL((ku3d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), grid, dim3(BS), lds,
+                       P->stream, a);
     if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
       TL_CHECK_LAUNCH();
-      return launch_fold(const_cast<tl_problem*>(P), grid.x * grid.y, a.fin, a.dot_out, a.done);
+      return launch_fold(const_cast<tl_problem*>(P), grid.x, a.fin, a.dot_out, a.done);
     }
   } else {
-    a.zchunk = 1;
+    a.ncol = 0;
     if constexpr (NDPN <= 2 && FAST) {
       switch (P->op.tile2d[sizeof(T) == 8 ? 0 : 1]) {
         case 16: return launch_ku2t<NDPN, T, MODE, TB, TO, FUSEP, 16>(P, a);
