@@ -341,36 +341,14 @@ __device__ inline void dma_plane(const KuArgs& a, const void* src, T* lds, int p
 // LDS copy is pending afterwards.
 __device__ inline void wait_copies() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// FUSEP: form the new search direction z + beta*p_old in place on the
-// entries that this wave copied (dwords [r*BS + 64*wave, +64) of every
-// round r: whole entries); call after wait_copies().  The barrier that
-// precedes the plane's first read publishes the result, so no extra
-// barrier is needed.
-template <int NDPN, typename T>
-__device__ inline void combine_own(const KuArgs& a, T* lz, const T* lp, T beta) {
-  constexpr int EPW = 64 * PlaneDma<NDPN, T>::UB / int(sizeof(T));  // entries per wave copy
-  const int n = (BS + 2 * a.ny + 2) * NDPN;
-  const int nu = PlaneDma<NDPN, T>::units(a.ny);
-  const int lane = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  if (lane < EPW) {
-    for (int r = 0; r * BS < nu; ++r) {
-      const int e = (r * BS + wave * 64) * PlaneDma<NDPN, T>::UB / int(sizeof(T)) + lane;
-      if (e < n) lz[e] = lz[e] + beta * lp[e];
-    }
-  }
-}
-
-// LDS plane buffers of one workgroup: z(parity) for the operator's input
-// (with FUSEP: z, overwritten by the new direction z + beta*p_old) and, with
-// FUSEP, p(parity) for the previous search direction.  Addresses are formed
-// arithmetically (a runtime-indexed pointer array would live in scratch).
+// LDS plane buffers of one workgroup, one per plane parity.  Addresses are
+// formed arithmetically (a runtime-indexed pointer array would live in
+// scratch).
 template <typename T>
 struct PlaneBufs {
   char* base;
   unsigned bb;  // bytes per buffer
   __device__ T* z(int parity) const { return reinterpret_cast<T*>(base + parity * bb); }
-  __device__ T* p(int parity) const { return reinterpret_cast<T*>(base + (2 + parity) * bb); }
 };
 
 // Epilogue inputs of one node, fetched at the start of its step.
@@ -478,18 +456,14 @@ __device__ inline void layer_sum(const KuArgs& a, const T (&U)[2][9][NDPN], cons
 // The step reads plane iz+1, starts the copy of plane iz+2 (when a later
 // step needs it) into the buffer plane iz came from, prefetches layer iz+1's
 // densities, completes and writes node iz and carries node iz+1's partial sum.
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST,
-          int SA, int SB>
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FAST, int SA, int SB>
 __device__ inline void z_step(const KuArgs& a, const PlaneBufs<T>& pb, int iz, int z1,
                               int64_t j0, int64_t j, int ix, int iy, bool active,
                               T (&U)[2][9][NDPN], T (&S)[4], T (&carry)[NDPN], double& dot) {
   const int pc = (iz + 1) & 1;  // buffer parity of plane iz+1 (plane iz+2: the other)
   __syncthreads();
   read_nbrs<NDPN, T, SB>(a, pb.z(pc), U);
-  if (iz + 1 < z1) {
-    dma_plane<NDPN, T>(a, a.u, pb.z(pc ^ 1), iz + 2, j0);
-    if constexpr (FUSEP) dma_plane<NDPN, T>(a, a.pin, pb.p(pc ^ 1), iz + 2, j0);
-  }
+  if (iz + 1 < z1) dma_plane<NDPN, T>(a, a.u, pb.z(pc ^ 1), iz + 2, j0);
   const int64_t node = int64_t(iz) * a.plane + (active ? j : 0);
   EpiPrefetch<NDPN, T, MODE, TB> ep;
   fetch_epi<NDPN, T, MODE, TB>(a, node, ep);
@@ -500,26 +474,19 @@ __device__ inline void z_step(const KuArgs& a, const PlaneBufs<T>& pb, int iz, i
   load_dens<T>(a, ix, iy, min(iz + 1, a.nelz), Sn);
   T acc_top[NDPN], acc_bot[NDPN];
   layer_sum<NDPN, T, SA, SB, FAST, true, true>(a, U, S, acc_top, acc_bot);
-  // Wait for the copies of plane iz+2 and the epilogue inputs here, after
-  // the FMAs (the empty asm consumes the sums, so the FMAs cannot be sunk
-  // below the wait) and before the stores: no copy is then pending at the
-  // next barrier, and the stores drain across it.
+  // Wait for the copy of plane iz+2 and the epilogue inputs here, after the
+  // FMAs (the empty asm consumes the sums, so the FMAs cannot be sunk below
+  // the wait) and before the stores: no copy is then pending at the next
+  // barrier, and the stores drain across it.
 #pragma unroll
   for (int d = 0; d < NDPN; ++d) asm volatile("" ::"v"(acc_top[d]), "v"(acc_bot[d]));
   wait_copies();
-  if constexpr (FUSEP) {
-    if (iz + 1 < z1) combine_own<NDPN, T>(a, pb.z(pc ^ 1), pb.p(pc ^ 1), T(*a.beta));
-  }
   if (active) {
     T uc[NDPN], tot[NDPN];
 #pragma unroll
     for (int d = 0; d < NDPN; ++d) {
       uc[d] = U[SA][4][d];
       tot[d] = carry[d] + acc_top[d];
-    }
-    if constexpr (FUSEP) {
-#pragma unroll
-      for (int d = 0; d < NDPN; ++d) static_cast<T*>(a.pout)[node * NDPN + d] = uc[d];
     }
     ku_epilogue_pf<NDPN, T, MODE, TB, TO>(a, node, ep, uc, tot, dot);
   }
@@ -532,7 +499,7 @@ __device__ inline void z_step(const KuArgs& a, const PlaneBufs<T>& pb, int iz, i
 // Node planes z0..z1-1 of the column of tile bx (BS consecutive nodes of a
 // plane, starting at node bx*BS).  Called with the LDS free (all threads
 // past their last read of the previous segment, no copy in flight).
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FAST>
 __device__ inline void ku3d_segment(const KuArgs& a, const PlaneBufs<T>& pb, unsigned bx, int z0,
                                     int z1, double& dot) {
   const int64_t j0 = int64_t(bx) * BS;
@@ -544,22 +511,12 @@ __device__ inline void ku3d_segment(const KuArgs& a, const PlaneBufs<T>& pb, uns
   T S[4];
   T carry[NDPN];
   // prologue: planes z0-1 -> U[0], z0 -> U[1]; layer z0-1 gives node z0's
-  // "below" part; the copy of plane z0+1 starts for the first step; layer
-  // z0 densities
+  // "below" part; the copy of plane z0+1 for the first step; layer z0
+  // densities
   const int pm = (z0 + 1) & 1;  // parity of z0-1 == parity of z0+1
   dma_plane<NDPN, T>(a, a.u, pb.z(pm), z0 - 1, j0);
   dma_plane<NDPN, T>(a, a.u, pb.z(pm ^ 1), z0, j0);
-  if constexpr (FUSEP) {
-    dma_plane<NDPN, T>(a, a.pin, pb.p(pm), z0 - 1, j0);
-    dma_plane<NDPN, T>(a, a.pin, pb.p(pm ^ 1), z0, j0);
-  }
   load_dens<T>(a, ix, iy, z0 - 1, S);
-  wait_copies();
-  if constexpr (FUSEP) {
-    const T beta = T(*a.beta);
-    combine_own<NDPN, T>(a, pb.z(pm), pb.p(pm), beta);
-    combine_own<NDPN, T>(a, pb.z(pm ^ 1), pb.p(pm ^ 1), beta);
-  }
   __syncthreads();
   read_nbrs<NDPN, T, 0>(a, pb.z(pm), U);
   read_nbrs<NDPN, T, 1>(a, pb.z(pm ^ 1), U);
@@ -569,18 +526,16 @@ __device__ inline void ku3d_segment(const KuArgs& a, const PlaneBufs<T>& pb, uns
   }
   __syncthreads();  // every wave has read plane z0-1 before its buffer is refilled
   dma_plane<NDPN, T>(a, a.u, pb.z(pm), z0 + 1, j0);
-  if constexpr (FUSEP) dma_plane<NDPN, T>(a, a.pin, pb.p(pm), z0 + 1, j0);
   load_dens<T>(a, ix, iy, z0, S);
   // no copy may be pending where the step loop starts: the compiler would
   // otherwise wait for every memory operation at the loop's barrier, the
   // previous step's stores included
   wait_copies();
-  if constexpr (FUSEP) combine_own<NDPN, T>(a, pb.z(pm), pb.p(pm), T(*a.beta));
   // alternating U slots: plane iz in slot SA, plane iz+1 read into SB
   for (int iz = z0; iz < z1; iz += 2) {
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 1, 0>(a, pb, iz, z1, j0, j, ix, iy, inplane, U, S, carry, dot);
+    z_step<NDPN, T, MODE, TB, TO, FAST, 1, 0>(a, pb, iz, z1, j0, j, ix, iy, inplane, U, S, carry, dot);
     if (iz + 1 >= z1) break;
-    z_step<NDPN, T, MODE, TB, TO, FUSEP, FAST, 0, 1>(a, pb, iz + 1, z1, j0, j, ix, iy, inplane, U, S, carry, dot);
+    z_step<NDPN, T, MODE, TB, TO, FAST, 0, 1>(a, pb, iz + 1, z1, j0, j, ix, iy, inplane, U, S, carry, dot);
   }
 }
 
@@ -589,7 +544,7 @@ __device__ inline void ku3d_segment(const KuArgs& a, const PlaneBufs<T>& pb, uns
 // split evenly over the grid in column-major order, so every workgroup
 // walks about the same number of planes, in one or two column segments.
 // A grid larger than the resident slots would leave a partial last round.
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FAST>
 __global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
   if (a.done && *a.done) return;
   extern __shared__ double lds_raw[];
@@ -614,11 +569,11 @@ __global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
   const unsigned bx = __builtin_amdgcn_readfirstlane(u0 / unsigned(a.nz));
   const int z0 = int(u0 - bx * unsigned(a.nz));
   const int z1 = min(a.nz, z0 + int(u1 - u0));
-  ku3d_segment<NDPN, T, MODE, TB, TO, FUSEP, FAST>(a, pb, bx, z0, z1, dot);
+  ku3d_segment<NDPN, T, MODE, TB, TO, FAST>(a, pb, bx, z0, z1, dot);
   const int rest = int(u1 - u0) - (z1 - z0);
   if (rest > 0) {
     __syncthreads();  // the next segment's prologue reuses both LDS buffers
-    ku3d_segment<NDPN, T, MODE, TB, TO, FUSEP, FAST>(a, pb, bx + 1, 0, rest, dot);
+    ku3d_segment<NDPN, T, MODE, TB, TO, FAST>(a, pb, bx + 1, 0, rest, dot);
   }
   if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
     __shared__ double sh[BS / 64 + 2];
@@ -794,13 +749,19 @@ static int launch_ku_impl(const tl_problem* P, KuArgs a) {
   const int L = BS + 2 * g.ny + 2;
   const unsigned nbx = unsigned((int64_t(a.plane) + BS - 1) / BS);
   if (g.dim == 3) {
+    // p = z + beta p is a separate pass before the 3D product (the fused
+    // form needed a second copy stream and an in-LDS combine and was
+    // slower: 96 vs 64 + 19 us per product at 192x96x96)
+    if constexpr (FUSEP) {
+      set_error("internal: the 3D operator has no fused search-direction update");
+      return TL_EINVAL;
+    } else {
     // one workgroup per resident slot, at least 4 planes each
-    const size_t lds = (FUSEP ? 4 : 2) * PlaneDma<NDPN, T>::buf_bytes(g.ny);
+    const size_t lds = 2 * PlaneDma<NDPN, T>::buf_bytes(g.ny);
     TL_ARG(lds <= size_t(160) * 1024, "grid too wide in y for the 3D operator's LDS planes");
     const int64_t units = int64_t(nbx) * g.nz;
     int64_t slots = 0;
-    TL_TRY(resident_slots(reinterpret_cast<const void*>(
-                              ku3d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>),
+    TL_TRY(resident_slots(reinterpret_cast<const void*>(ku3d_kernel<NDPN, T, MODE, TB, TO, FAST>),
                           lds, slots));
     // at least one workgroup per column: a range of <= nz units spans at
     // most two columns, which is what the kernel walks
@@ -814,11 +775,12 @@ static int launch_ku_impl(const tl_problem* P, KuArgs a) {
     }
     a.ncol = int(nbx);
     const dim3 grid(static_cast<unsigned>(nwg));
-    hipLaunchKernelGGL((ku3d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), grid, dim3(BS), lds,
+    hipLaunchKernelGGL((ku3d_kernel<NDPN, T, MODE, TB, TO, FAST>), grid, dim3(BS), lds,
                        P->stream, a);
     if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
       TL_CHECK_LAUNCH();
       return launch_fold(const_cast<tl_problem*>(P), grid.x, a.fin, a.dot_out, a.done);
+    }
     }
   } else {
     a.ncol = 0;
@@ -1462,6 +1424,18 @@ int pcg_init(tl_problem* P, const PcgCtx& c, const double* f, const double* x0, 
   return TL_OK;
 }
 
+// p_out = z + beta p_in (the 3D search-direction update, before K p)
+__global__ __launch_bounds__(BS) void pupdate_kernel(const double* __restrict__ z,
+                                                     const double* __restrict__ pin,
+                                                     double* __restrict__ pout,
+                                                     const double* __restrict__ beta, int64_t n,
+                                                     const int* done) {
+  if (*done) return;
+  const double b = *beta;
+  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS)
+    pout[i] = z[i] + b * pin[i];
+}
+
 // One PCG iteration (k counts from 0 after pcg_init).  Every kernel returns
 // at once when the device state says done, so iterations launched past
 // convergence change nothing.  No allocation, memset or host transfer: the
@@ -1478,12 +1452,21 @@ int pcg_iteration(tl_problem* P, const PcgCtx& c, double* x, int precond, int64_
     TL_CHECK_LAUNCH();
     TL_TRY(launch_fold(P, c.rb, FIN_RHO, nullptr, c.done));
   }
-  // p = z + beta p ; q = K p ; pq ; alpha.  The fused product reads p_k
-  // (halo included) while it writes p_{k+1}: two buffers.
+  // p = z + beta p ; q = K p ; pq ; alpha.  The fused product (2D, and the
+  // 3D MFMA operator) reads p_k (halo included) while it writes p_{k+1}:
+  // two buffers.  The 3D vector operator takes the update as a separate pass.
   double* pin = c.pp[k & 1];
   double* pout = c.pp[(k + 1) & 1];
-  TL_TRY(fine_apply_ex(P, c.z, c.q, TL_PREC_FP64, KU_STORE_DOT, nullptr, false, nullptr, 0.0, pin,
-                       pout, &c.st->beta, nullptr, FIN_PQ, c.done, false));
+  if (P->g.dim == 3 && !(P->op.mfma64 && P->ndpn == 3)) {
+    hipLaunchKernelGGL(pupdate_kernel, dim3(c.nb), dim3(BS), 0, s, c.z, pin, pout, &c.st->beta,
+                       c.n, c.done);
+    TL_CHECK_LAUNCH();
+    TL_TRY(fine_apply_ex(P, pout, c.q, TL_PREC_FP64, KU_STORE_DOT, nullptr, false, nullptr, 0.0,
+                         nullptr, nullptr, nullptr, nullptr, FIN_PQ, c.done, false));
+  } else {
+    TL_TRY(fine_apply_ex(P, c.z, c.q, TL_PREC_FP64, KU_STORE_DOT, nullptr, false, nullptr, 0.0,
+                         pin, pout, &c.st->beta, nullptr, FIN_PQ, c.done, false));
+  }
   // x += alpha p ; r -= alpha q ; rr ; done
   hipLaunchKernelGGL(update_kernel, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n, c.partials,
                      c.st, c.done);
