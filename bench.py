@@ -45,8 +45,8 @@ def log(*a):
 def solve_traffic_bytes(dims, nlevels, iters, vprec_bytes=4):
     """Algorithmic HBM bytes of one solve (each array touched once per kernel).
 
-    Fine level per PCG iteration: fused p/q product 32 B/DOF + 8 B/el, x/r
-    update 48 B/DOF; V-cycle fine level: zero-guess smoothing (r, dinv -> x),
+    Fine level per PCG iteration: p = z + beta p 24 B/DOF, product q = K p
+    16 B/DOF + 8 B/el, x/r update 48 B/DOF; V-cycle fine level: zero-guess smoothing (r, dinv -> x),
     residual (x, r -> r0), prolongation-add (x), final smoothing writing z in
     FP64 (x, r, dinv -> z).  Coarse levels: stencil bytes twice (residual and
     smoothing) plus their vectors; coarsest dense inverse once.
@@ -56,7 +56,7 @@ def solve_traffic_bytes(dims, nlevels, iters, vprec_bytes=4):
     nel = nelx * nely * nelz
     ndof = ndpn * (nelx + 1) * (nely + 1) * (nelz + 1)
     w = vprec_bytes
-    fine_pcg = 32 * ndof + 8 * nel + 48 * ndof
+    fine_pcg = 24 * ndof + 16 * ndof + 8 * nel + 48 * ndof
     fine_v = (8 + w + w) * ndof + (w + 8 + w) * ndof + w * nel + 2 * w * ndof \
         + (w + 8 + w + 8) * ndof + w * nel
     coarse = 0
@@ -116,9 +116,12 @@ def kernel_kinds(dims, vcycle, nlevels, fp64_mfma=False):
         vbound, vpeak = "mfma", FP32_PEAK_TFLOPS
     sb = 14 * 9 * nn1 * vb  # level-1 block stencil bytes (forward half: centre + 13 blocks)
     return {
+        # the MFMA walk fuses p = z + beta p (reads z and p, writes p and q);
+        # the vector operator takes p from a separate pointwise pass
         0: ("kucol64_kernel FP64 MFMA fused PCG product (p = z + beta p; q = K p; p.q)" if fp64_mfma
-            else "ku3d_kernel FP64 fused PCG product (p = z + beta p; q = K p; p.q)",
-            32 * ndof + 8 * nel, flops_ku, "mfma", FP64_PEAK_TFLOPS),
+            else "ku3d_kernel FP64 PCG product (q = K p; p.q)",
+            (32 if fp64_mfma else 16) * ndof + 8 * nel + ndof // 3, flops_ku, "mfma",
+            FP64_PEAK_TFLOPS),
         1: (f"{vk} V-cycle residual (r = b - K x)",
             (2 * vb + 8) * ndof + vb * nel + ndof // 3, flops_ku, vbound, vpeak),
         2: (f"{vk} V-cycle damped-Jacobi sweep (z in FP64, r.z)",
