@@ -1059,10 +1059,16 @@ __global__ __launch_bounds__(BS) void resid_kernel(const double* __restrict__ b,
 }
 
 // x += alpha p ; r -= alpha q ; rr
+// x += alpha p ; r -= alpha q ; r.r partials.  With xs (GMG with V(1,1)),
+// also the next V-cycle's first fine-level sweep xs = omega * dinv * r from
+// the new residual, in the V-cycle's precision TS (saves that sweep's pass).
+template <typename TS>
 __global__ __launch_bounds__(BS) void update_kernel(double* __restrict__ x, double* __restrict__ r,
                                                     const double* __restrict__ p,
                                                     const double* __restrict__ q, int64_t n,
                                                     double* partials, const PcgState* st,
+                                                    TS* __restrict__ xs,
+                                                    const TS* __restrict__ dinv, TS omega,
                                                     const int* done) {
   if (*done) return;
   const double alpha = st->alpha;
@@ -1072,9 +1078,19 @@ __global__ __launch_bounds__(BS) void update_kernel(double* __restrict__ x, doub
     const double rv = r[i] - alpha * q[i];
     r[i] = rv;
     s += rv * rv;
+    if (xs) xs[i] = zero_guess_sweep<TS, double>(omega, dinv[i], rv);
   }
   double v[1] = {s};
   reduce_store<1>(v, partials);
+}
+
+// xs = omega * dinv * r (the first fine-level sweep of the first V-cycle)
+template <typename TS>
+__global__ __launch_bounds__(BS) void zero_sweep_kernel(const double* __restrict__ r,
+                                                        const TS* __restrict__ dinv, TS omega,
+                                                        TS* __restrict__ xs, int64_t n) {
+  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS)
+    xs[i] = zero_guess_sweep<TS, double>(omega, dinv[i], r[i]);
 }
 
 // z = dinv * r (or z = r when dinv == nullptr); rho = r.z
@@ -1395,6 +1411,14 @@ int pcg_context(tl_problem* P, PcgCtx& c) {
 }
 
 // t = masked f, bb, state, x = masked x0 or 0, r = t - K x, rr0, p = 0.
+// GMG-PCG with the default V(1,1) cycle: the residual update writes the next
+// V-cycle's first fine-level sweep (the fine iterate buffer never rotates
+// with one pre- and one post-smoothing sweep).
+bool pcg_fuses_x0(const tl_problem* P, int precond) {
+  return precond == TL_PRECOND_GMG && P->mg.ready && P->mg.nu1 == 1 && P->mg.nu2 == 1 &&
+         P->mg.nlevels >= 2 && !P->mg.lv.empty() && P->mg.lv[0].vx.ptr != nullptr;
+}
+
 int pcg_init(tl_problem* P, const PcgCtx& c, const double* f, const double* x0, double* x,
              const tl_solve_opts* o) {
   hipStream_t s = P->stream;
@@ -1421,6 +1445,16 @@ int pcg_init(tl_problem* P, const PcgCtx& c, const double* f, const double* x0, 
   TL_TRY(launch_fold(P, c.rb, FIN_RR0, nullptr, nullptr));
   TL_HIP(hipMemsetAsync(c.pp[0], 0, sizeof(double) * n, s));
   TL_HIP(hipMemsetAsync(c.pp[1], 0, sizeof(double) * n, s));
+  if (pcg_fuses_x0(P, o->precond)) {
+    const MgState& M = P->mg;
+    if (M.prec == TL_PREC_FP64)
+      hipLaunchKernelGGL(zero_sweep_kernel<double>, dim3(c.nb), dim3(BS), 0, s, c.r,
+                         P->op.dinv64.as<double>(), M.omega, M.lv[0].vx.as<double>(), n);
+    else
+      hipLaunchKernelGGL(zero_sweep_kernel<float>, dim3(c.nb), dim3(BS), 0, s, c.r,
+                         P->op.dinv32.as<float>(), float(M.omega), M.lv[0].vx.as<float>(), n);
+    TL_CHECK_LAUNCH();
+  }
   return TL_OK;
 }
 
@@ -1443,8 +1477,9 @@ __global__ __launch_bounds__(BS) void pupdate_kernel(const double* __restrict__ 
 int pcg_iteration(tl_problem* P, const PcgCtx& c, double* x, int precond, int64_t k) {
   hipStream_t s = P->stream;
   // z = M r ; rho = r.z ; beta
+  const bool fx0 = pcg_fuses_x0(P, precond);
   if (precond == TL_PRECOND_GMG) {
-    TL_TRY(mg_vcycle(P, c.r, c.z, c.done));
+    TL_TRY(mg_vcycle(P, c.r, c.z, c.done, fx0));
   } else {
     hipLaunchKernelGGL(jacobi_prec_kernel, dim3(c.rb), dim3(BS), 0, s, c.r,
                        precond == TL_PRECOND_JACOBI ? P->op.dinv64.as<double>() : nullptr, c.z,
@@ -1468,8 +1503,18 @@ int pcg_iteration(tl_problem* P, const PcgCtx& c, double* x, int precond, int64_
                          pin, pout, &c.st->beta, nullptr, FIN_PQ, c.done, false));
   }
   // x += alpha p ; r -= alpha q ; rr ; done
-  hipLaunchKernelGGL(update_kernel, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n, c.partials,
-                     c.st, c.done);
+  if (fx0 && P->mg.prec == TL_PREC_FP64)
+    hipLaunchKernelGGL(update_kernel<double>, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n,
+                       c.partials, c.st, P->mg.lv[0].vx.as<double>(), P->op.dinv64.as<double>(),
+                       P->mg.omega, c.done);
+  else if (fx0)
+    hipLaunchKernelGGL(update_kernel<float>, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n,
+                       c.partials, c.st, P->mg.lv[0].vx.as<float>(), P->op.dinv32.as<float>(),
+                       float(P->mg.omega), c.done);
+  else
+    hipLaunchKernelGGL(update_kernel<double>, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n,
+                       c.partials, c.st, static_cast<double*>(nullptr),
+                       static_cast<const double*>(nullptr), 0.0, c.done);
   TL_CHECK_LAUNCH();
   return launch_fold(P, c.rb, FIN_RR, nullptr, c.done);
 }

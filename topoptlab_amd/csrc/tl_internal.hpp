@@ -247,7 +247,9 @@ int fine_diag(tl_problem* P);
 // launch timing (tl_profile): no-ops while profiling is off
 int prof_begin(tl_problem* P, int kind, ProfEvent& ev);
 int prof_end(tl_problem* P, ProfEvent& ev);
-int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done);
+// x0_ready: the fine level's first (zero-guess) smoothing sweep is already
+// in lv[0].vx (written by the PCG's residual update, see pcg_fuses_x0)
+int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done, bool x0_ready);
 // V-cycle fine operator on MFMA (3D H8 elasticity, prec TL_PREC_BF16/FP16)
 int fine_apply_mfma(tl_problem* P, const void* u, void* y, int prec, int mode, const void* b,
                     const void* dinv, double omega, double* partials, const int* done,
@@ -258,6 +260,14 @@ int fine_apply_mfma64(tl_problem* P, const double* u, double* y, int mode, const
                       const double* beta, double* partials, const int* done,
                       unsigned* nblocks_out);
 }  // namespace tl
+
+// First damped-Jacobi sweep from a zero guess, x = omega * dinv * b, in the
+// V-cycle's precision T (the fine level's b is the fp64 PCG residual).  One
+// definition for every kernel that forms it, so their results agree bit for bit.
+template <typename T, typename TB>
+__device__ inline T zero_guess_sweep(T omega, T dinv, TB b) {
+  return omega * dinv * T(b);
+}
 
 // modes of the fine matrix-free operator epilogue
 enum {

@@ -294,7 +294,7 @@ __global__ void jacobi_zero_kernel(const TB* __restrict__ b, const T* __restrict
   if (*done) return;
   for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < n;
        i += int64_t(gridDim.x) * blockDim.x)
-    x[i] = omega * dinv[i] * T(b[i]);
+    x[i] = zero_guess_sweep<T, TB>(omega, dinv[i], b[i]);
 }
 
 // coarse b = P^T r (masked)
@@ -681,7 +681,7 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
 
 // Fine level: r (fp64 PCG residual) -> z (fp64), rho = r.z into the PCG state.
 template <int NDPN, typename T>
-int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done) {
+int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done, bool x0_ready) {
   MgState& M = P->mg;
   TL_ARG(M.nlevels >= 2 && int(M.lv.size()) == M.nlevels, "internal: hierarchy not set up");
   Level& L = M.lv[0];
@@ -697,9 +697,11 @@ int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done) {
   const void* dinv = std::is_same<T, double>::value ? P->op.dinv64.ptr : P->op.dinv32.ptr;
   T* x = vp<T>(L.vx);
   T* t = vp<T>(L.vt);
-  hipLaunchKernelGGL((jacobi_zero_kernel<T, double>), dim3(gsz(n)), dim3(BS), 0, P->stream, r,
-                     static_cast<const T*>(dinv), T(M.omega), x, n, done);
-  TL_CHECK_LAUNCH();
+  if (!x0_ready) {
+    hipLaunchKernelGGL((jacobi_zero_kernel<T, double>), dim3(gsz(n)), dim3(BS), 0, P->stream, r,
+                       static_cast<const T*>(dinv), T(M.omega), x, n, done);
+    TL_CHECK_LAUNCH();
+  }
   for (int s = 1; s < M.nu1; ++s) {
     TL_TRY(fine_apply_ex(P, x, t, prec, KU_JACOBI, r, true, dinv, M.omega, nullptr, nullptr,
                          nullptr, nullptr, 0, done, false));
@@ -731,12 +733,17 @@ int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done) {
 
 }  // namespace
 
-int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done) {
+int mg_vcycle(tl_problem* P, const double* r, double* z, const int* done, bool x0_ready) {
   const bool f64 = P->mg.prec == TL_PREC_FP64;
+  TL_ARG(!x0_ready || (P->mg.nu1 == 1 && P->mg.nu2 == 1),
+         "internal: a precomputed first sweep needs V(1,1) (the iterate buffer must not rotate)");
   switch (P->ndpn) {
-    case 1: return f64 ? vcycle_fine<1, double>(P, r, z, done) : vcycle_fine<1, float>(P, r, z, done);
-    case 2: return f64 ? vcycle_fine<2, double>(P, r, z, done) : vcycle_fine<2, float>(P, r, z, done);
-    case 3: return f64 ? vcycle_fine<3, double>(P, r, z, done) : vcycle_fine<3, float>(P, r, z, done);
+    case 1: return f64 ? vcycle_fine<1, double>(P, r, z, done, x0_ready)
+                       : vcycle_fine<1, float>(P, r, z, done, x0_ready);
+    case 2: return f64 ? vcycle_fine<2, double>(P, r, z, done, x0_ready)
+                       : vcycle_fine<2, float>(P, r, z, done, x0_ready);
+    case 3: return f64 ? vcycle_fine<3, double>(P, r, z, done, x0_ready)
+                       : vcycle_fine<3, float>(P, r, z, done, x0_ready);
   }
   set_error("unsupported DOFs per node");
   return TL_EINVAL;
