@@ -46,7 +46,8 @@ def solve_traffic_bytes(dims, nlevels, iters, vprec_bytes=4):
     """Algorithmic HBM bytes of one solve (each array touched once per kernel).
 
     Fine level per PCG iteration: p = z + beta p 24 B/DOF, product q = K p
-    16 B/DOF + 8 B/el, x/r update 48 B/DOF; V-cycle fine level: zero-guess smoothing (r, dinv -> x),
+    16 B/DOF + 8 B/el, x/r update 48 B/DOF plus the next V-cycle's
+    zero-guess sweep (dinv -> x) that it writes; V-cycle fine level:
     residual (x, r -> r0), prolongation-add (x), final smoothing writing z in
     FP64 (x, r, dinv -> z).  Coarse levels: stencil bytes twice (residual and
     smoothing) plus their vectors; coarsest dense inverse once.
@@ -56,8 +57,8 @@ def solve_traffic_bytes(dims, nlevels, iters, vprec_bytes=4):
     nel = nelx * nely * nelz
     ndof = ndpn * (nelx + 1) * (nely + 1) * (nelz + 1)
     w = vprec_bytes
-    fine_pcg = 24 * ndof + 16 * ndof + 8 * nel + 48 * ndof
-    fine_v = (8 + w + w) * ndof + (w + 8 + w) * ndof + w * nel + 2 * w * ndof \
+    fine_pcg = 24 * ndof + 16 * ndof + 8 * nel + (48 + 2 * w) * ndof
+    fine_v = (w + 8 + w) * ndof + w * nel + 2 * w * ndof \
         + (w + 8 + w + 8) * ndof + w * nel
     coarse = 0
     d = [nelx, nely, nelz]
