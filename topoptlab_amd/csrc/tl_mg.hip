@@ -297,8 +297,11 @@ __global__ void jacobi_zero_kernel(const TB* __restrict__ b, const T* __restrict
     x[i] = zero_guess_sweep<T, TB>(omega, dinv[i], b[i]);
 }
 
-// coarse b = P^T r (masked)
-template <int NDPN, typename T>
+// coarse b = P^T r (masked).  The 27 (2D: 9) fine neighbours are read from
+// clamped positions, those outside the grid with weight 0, so every load is
+// issued before the first sum (a loop that stepped over them waited for each
+// load in turn); the summation order is unchanged.
+template <int NDPN, typename T, bool D3>
 __global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
                                 const uint8_t* __restrict__ maskc, T* __restrict__ bc,
                                 const int* done) {
@@ -310,14 +313,19 @@ __global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
   T acc[NDPN];
 #pragma unroll
   for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
-  const bool d3 = gc.dim == 3;
-  for (int dz = (d3 ? -1 : 0); dz <= (d3 ? 1 : 0); ++dz)
+#pragma unroll
+  for (int dz = (D3 ? -1 : 0); dz <= (D3 ? 1 : 0); ++dz)
+#pragma unroll
     for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
       for (int dy = -1; dy <= 1; ++dy) {
-        const int fx = 2 * ix + dx, fy = 2 * iy + dy, fz = d3 ? 2 * iz + dz : 0;
-        if (!node_in(gf, fx, fy, fz)) continue;
-        const T w = T((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0));
-        const int64_t fi = node_id(gf, fx, fy, fz);
+        const int fx = 2 * ix + dx, fy = 2 * iy + dy, fz = D3 ? 2 * iz + dz : 0;
+        const bool ok = node_in(gf, fx, fy, fz);
+        const int64_t fi = node_id(gf, min(max(fx, 0), gf.nx - 1), min(max(fy, 0), gf.ny - 1),
+                                   min(max(fz, 0), gf.nz - 1));
+        // a zero weight (not a branch) for neighbours outside the grid, so
+        // that the loads stay unconditional; their clamped values are finite
+        const T w = ok ? T((dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0)) : T(0);
 #pragma unroll
         for (int d = 0; d < NDPN; ++d) acc[d] += w * rf[fi * NDPN + d];
       }
@@ -326,8 +334,9 @@ __global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
   for (int d = 0; d < NDPN; ++d) bc[n * NDPN + d] = ((mk >> d) & 1) ? T(0) : acc[d];
 }
 
-// fine x += P xc (masked)
-template <int NDPN, typename T>
+// fine x += P xc (masked).  The up to 8 coarse nodes are read from clamped
+// positions, all loads before the first sum, the unused ones with weight 0.
+template <int NDPN, typename T, bool D3>
 __global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
                                    const uint8_t* __restrict__ maskf, T* __restrict__ xf,
                                    const int* done) {
@@ -342,11 +351,16 @@ __global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
   const int x0 = ix >> 1, x1 = (ix + 1) >> 1, y0 = iy >> 1, y1 = (iy + 1) >> 1;
   const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
   const T wx = (ix & 1) ? T(0.5) : T(1), wy = (iy & 1) ? T(0.5) : T(1), wz = (iz & 1) ? T(0.5) : T(1);
-  for (int cz = z0; cz <= z1; ++cz)
-    for (int cx = x0; cx <= x1; ++cx)
-      for (int cy = y0; cy <= y1; ++cy) {
-        const T w = wx * wy * wz;
-        const int64_t ci = node_id(gc, cx, cy, cz);
+#pragma unroll
+  for (int a = 0; a < (D3 ? 2 : 1); ++a)
+#pragma unroll
+    for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+      for (int by = 0; by < 2; ++by) {
+        const int cz = z0 + a, cx = x0 + bx, cy = y0 + by;
+        const bool ok = cz <= z1 && cx <= x1 && cy <= y1;
+        const int64_t ci = node_id(gc, min(cx, gc.nx - 1), min(cy, gc.ny - 1), min(cz, gc.nz - 1));
+        const T w = ok ? wx * wy * wz : T(0);
 #pragma unroll
         for (int d = 0; d < NDPN; ++d) acc[d] += w * xc[ci * NDPN + d];
       }
@@ -354,6 +368,34 @@ __global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
 #pragma unroll
   for (int d = 0; d < NDPN; ++d)
     if (!((mk >> d) & 1)) xf[n * NDPN + d] += acc[d];
+}
+
+template <int NDPN, typename T>
+int launch_restrict(tl_problem* P, const Grid& gf, const Grid& gc, const T* rf,
+                    const uint8_t* maskc, T* bc, const int* done) {
+  const dim3 grid(unsigned((gc.nnodes + BS - 1) / BS));
+  if (gf.dim == 3)
+    hipLaunchKernelGGL((restrict_kernel<NDPN, T, true>), grid, dim3(BS), 0, P->stream, to_d(gf),
+                       to_d(gc), rf, maskc, bc, done);
+  else
+    hipLaunchKernelGGL((restrict_kernel<NDPN, T, false>), grid, dim3(BS), 0, P->stream, to_d(gf),
+                       to_d(gc), rf, maskc, bc, done);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+template <int NDPN, typename T>
+int launch_prolong(tl_problem* P, const Grid& gf, const Grid& gc, const T* xc,
+                   const uint8_t* maskf, T* xf, const int* done) {
+  const dim3 grid(unsigned((gf.nnodes + BS - 1) / BS));
+  if (gf.dim == 3)
+    hipLaunchKernelGGL((prolong_add_kernel<NDPN, T, true>), grid, dim3(BS), 0, P->stream, to_d(gf),
+                       to_d(gc), xc, maskf, xf, done);
+  else
+    hipLaunchKernelGGL((prolong_add_kernel<NDPN, T, false>), grid, dim3(BS), 0, P->stream,
+                       to_d(gf), to_d(gc), xc, maskf, xf, done);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
 }
 
 // x = Cinv b on the coarsest grid (Cinv fp64, padded leading dimension)
@@ -365,8 +407,24 @@ __global__ __launch_bounds__(BS) void dense_gemv_kernel(const double* __restrict
   const int64_t row = blockIdx.x * int64_t(BS / 64) + (threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
   if (row >= n) return;
+  const double* __restrict__ Ar = A + row * np;
   double s = 0.0;
-  for (int64_t j = lane; j < n; j += 64) s += A[row * np + j] * double(b[j]);
+  // U loads per lane in flight before the first FMA (a plain loop waited for
+  // each load: ~30 dependent round trips per row); the FMAs keep the column
+  // order j = lane, lane + 64, ...
+  constexpr int U = 8;
+  int64_t j = lane;
+  for (; j + 64 * (U - 1) < n; j += 64 * U) {
+    double a[U], v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      a[u] = Ar[j + 64 * u];
+      v[u] = double(b[j + 64 * u]);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += a[u] * v[u];
+  }
+  for (; j < n; j += 64) s += Ar[j] * double(b[j]);
   s = wave_sum(s);
   if (lane == 0) x[row] = T(s);
 }
@@ -660,15 +718,10 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
     }
     TL_TRY((st_apply<NDPN, T>(P, l, 0, x, b, vp<T>(L.vr), done)));
     Level& C = M.lv[l + 1];
-    hipLaunchKernelGGL((restrict_kernel<NDPN, T>), dim3(unsigned((C.g.nnodes + BS - 1) / BS)),
-                       dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(L.vr),
-                       C.mask.as<uint8_t>(), vp<T>(C.vb), done);
-    TL_CHECK_LAUNCH();
+    TL_TRY((launch_restrict<NDPN, T>(P, L.g, C.g, vp<T>(L.vr), C.mask.as<uint8_t>(), vp<T>(C.vb),
+                                      done)));
     TL_TRY((vcycle_level<NDPN, T>(P, l + 1, done)));
-    hipLaunchKernelGGL((prolong_add_kernel<NDPN, T>), dim3(unsigned((L.g.nnodes + BS - 1) / BS)),
-                       dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(C.vx),
-                       L.mask.as<uint8_t>(), x, done);
-    TL_CHECK_LAUNCH();
+    TL_TRY((launch_prolong<NDPN, T>(P, L.g, C.g, vp<T>(C.vx), L.mask.as<uint8_t>(), x, done)));
     for (int s = 0; s < M.nu2; ++s) {
       TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
       std::swap(L.vx.ptr, L.vt.ptr);
@@ -710,16 +763,11 @@ int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done, bool
   }
   TL_TRY(fine_apply_ex(P, x, vp<T>(L.vr), prec, KU_RESID, r, true, nullptr, 0.0, nullptr, nullptr,
                        nullptr, nullptr, 0, done, false));
-  hipLaunchKernelGGL((restrict_kernel<NDPN, T>), dim3(unsigned((C.g.nnodes + BS - 1) / BS)),
-                     dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(L.vr),
-                     C.mask.as<uint8_t>(), vp<T>(C.vb), done);
-  TL_CHECK_LAUNCH();
+  TL_TRY((launch_restrict<NDPN, T>(P, L.g, C.g, vp<T>(L.vr), C.mask.as<uint8_t>(), vp<T>(C.vb),
+                                    done)));
   TL_TRY((vcycle_level<NDPN, T>(P, 1, done)));
   // the fine grid's Dirichlet mask is the operator's (level 0 has no copy)
-  hipLaunchKernelGGL((prolong_add_kernel<NDPN, T>), dim3(unsigned((L.g.nnodes + BS - 1) / BS)),
-                     dim3(BS), 0, P->stream, to_d(L.g), to_d(C.g), vp<T>(C.vx),
-                     P->op.mask.as<uint8_t>(), x, done);
-  TL_CHECK_LAUNCH();
+  TL_TRY((launch_prolong<NDPN, T>(P, L.g, C.g, vp<T>(C.vx), P->op.mask.as<uint8_t>(), x, done)));
   for (int s = 0; s + 1 < M.nu2; ++s) {
     TL_TRY(fine_apply_ex(P, x, t, prec, KU_JACOBI, r, true, dinv, M.omega, nullptr, nullptr,
                          nullptr, nullptr, 0, done, false));
