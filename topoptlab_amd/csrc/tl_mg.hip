@@ -225,7 +225,48 @@ __global__ void st_finish_kernel(GridD g, int ndpn, const double* __restrict__ s
 // neighbours.  Workgroups take contiguous node ranges per XCD so that the
 // neighbours' blocks (up to one node plane back) are re-read from the XCD's
 // own L2.
-template <int NDPN, typename T, int MODE>
+// One 27-point (2D: 9-point) term A(n, m) x_m, m = n + (rx, ry, rz).  DIM is
+// the grid dimension when known at compile time (0: read it from g).
+template <int NDPN, typename T, int DIM>
+__device__ __forceinline__ void st_term(const GridD& g, const T* __restrict__ sth,
+                                        const T* __restrict__ x, int64_t n, int ix, int iy,
+                                        int iz, int rx, int ry, int rz, T (&acc)[NDPN]) {
+  const int dim = DIM ? DIM : g.dim;
+  const int kc = center_k(dim);
+  const int k = kidx(dim, rx, ry, rz);
+  // neighbours outside the grid carry zero weight: clamp the reads
+  const int mx = min(max(ix + rx, 0), g.nx - 1), my = min(max(iy + ry, 0), g.ny - 1),
+            mz = min(max(iz + rz, 0), g.nz - 1);
+  const int64_t m = node_id(g, mx, my, mz);
+  T xm[NDPN];
+#pragma unroll
+  for (int c = 0; c < NDPN; ++c) xm[c] = x[m * NDPN + c];
+  if (k >= kc) {  // forward block of node n (zero towards missing neighbours)
+    const T* h = sth + int64_t(k - kc) * NDPN * NDPN * g.nnodes + n;
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+      for (int c = 0; c < NDPN; ++c) acc[d] += h[int64_t(d * NDPN + c) * g.nnodes] * xm[c];
+  } else {  // A(n, m) = A(m, n)^T: forward block (offset -r) of neighbour m
+    const bool ok = node_in(g, ix + rx, iy + ry, iz + rz);
+    const T* h = sth + int64_t(2 * kc - k - kc) * NDPN * NDPN * g.nnodes + m;
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+      for (int c = 0; c < NDPN; ++c) {
+        const T w = h[int64_t(c * NDPN + d) * g.nnodes];
+        acc[d] += (ok ? w : T(0)) * xm[c];
+      }
+  }
+}
+
+// MODE 0: y = b - A x; 1: y = x + omega D^-1 (b - A x); fixed DOFs of y are 0.
+// DIMU 0: runtime dimension, a loop over the planes (large levels: fewer
+// registers, two waves per SIMD); 2 or 3: every term unrolled, so the
+// forward/backward choice is resolved at compile time and the loads of all
+// terms are in flight together (small levels, whose few waves cannot hide a
+// chain of dependent loads).  Same terms in the same order either way.
+template <int NDPN, typename T, int MODE, int DIMU>
 __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restrict__ sth,
                                                       const T* __restrict__ x,
                                                       const T* __restrict__ b,
@@ -243,39 +284,23 @@ __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restri
   T acc[NDPN];
 #pragma unroll
   for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
-  const bool d3 = g.dim == 3;
-  const int kc = center_k(g.dim);
-  for (int rz = (d3 ? -1 : 0); rz <= (d3 ? 1 : 0); ++rz)
+  if constexpr (DIMU == 0) {
+    const bool d3 = g.dim == 3;
+    for (int rz = (d3 ? -1 : 0); rz <= (d3 ? 1 : 0); ++rz)
 #pragma unroll
-    for (int rx = -1; rx <= 1; ++rx)
+      for (int rx = -1; rx <= 1; ++rx)
 #pragma unroll
-      for (int ry = -1; ry <= 1; ++ry) {
-        const int k = kidx(g.dim, rx, ry, rz);
-        // neighbours outside the grid carry zero weight: clamp the reads
-        const int mx = min(max(ix + rx, 0), g.nx - 1), my = min(max(iy + ry, 0), g.ny - 1),
-                  mz = min(max(iz + rz, 0), g.nz - 1);
-        const int64_t m = node_id(g, mx, my, mz);
-        T xm[NDPN];
+        for (int ry = -1; ry <= 1; ++ry)
+          st_term<NDPN, T, 0>(g, sth, x, n, ix, iy, iz, rx, ry, rz, acc);
+  } else {
 #pragma unroll
-        for (int c = 0; c < NDPN; ++c) xm[c] = x[m * NDPN + c];
-        if (k >= kc) {  // forward block of node n (zero towards missing neighbours)
-          const T* h = sth + int64_t(k - kc) * NDPN * NDPN * g.nnodes + n;
+    for (int rz = (DIMU == 3 ? -1 : 0); rz <= (DIMU == 3 ? 1 : 0); ++rz)
 #pragma unroll
-          for (int d = 0; d < NDPN; ++d)
+      for (int rx = -1; rx <= 1; ++rx)
 #pragma unroll
-            for (int c = 0; c < NDPN; ++c) acc[d] += h[int64_t(d * NDPN + c) * g.nnodes] * xm[c];
-        } else {  // A(n, m) = A(m, n)^T: forward block (offset -r) of neighbour m
-          const bool ok = node_in(g, ix + rx, iy + ry, iz + rz);
-          const T* h = sth + int64_t(2 * kc - k - kc) * NDPN * NDPN * g.nnodes + m;
-#pragma unroll
-          for (int d = 0; d < NDPN; ++d)
-#pragma unroll
-            for (int c = 0; c < NDPN; ++c) {
-              const T w = h[int64_t(c * NDPN + d) * g.nnodes];
-              acc[d] += (ok ? w : T(0)) * xm[c];
-            }
-        }
-      }
+        for (int ry = -1; ry <= 1; ++ry)
+          st_term<NDPN, T, DIMU>(g, sth, x, n, ix, iy, iz, rx, ry, rz, acc);
+  }
   const uint8_t mk = mask[n];
 #pragma unroll
   for (int d = 0; d < NDPN; ++d) {
@@ -666,12 +691,28 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
   ProfEvent ev{};
   ev.kind = -1;
   if (l == 1) TL_TRY(prof_begin(P, 3 + mode, ev));
-  if (mode == 0)
-    hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 0>), dim3(nb), dim3(BS), 0, P->stream, to_d(L.g),
-                       st, x, b, dinv, T(P->mg.omega), L.mask.as<uint8_t>(), y, done);
-  else
-    hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1>), dim3(nb), dim3(BS), 0, P->stream, to_d(L.g),
-                       st, x, b, dinv, T(P->mg.omega), L.mask.as<uint8_t>(), y, done);
+  // levels of up to 64k nodes (a few hundred workgroups) take the unrolled
+  // form; NDPN 2 only occurs in 2D and NDPN 3 in 3D for the unrolled forms
+  const int dimu = L.g.nnodes <= 65536 ? L.g.dim : 0;
+  const GridD gd = to_d(L.g);
+  const T om = T(P->mg.omega);
+  const uint8_t* mk = L.mask.as<uint8_t>();
+  auto go = [&](auto mode_c, auto dim_c) {
+    constexpr int MODE_ = decltype(mode_c)::value, DIM_ = decltype(dim_c)::value;
+    hipLaunchKernelGGL((st_apply_kernel<NDPN, T, MODE_, DIM_>), dim3(nb), dim3(BS), 0, P->stream,
+                       gd, st, x, b, dinv, om, mk, y, done);
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  if (dimu == 3 && NDPN != 2) {
+    if (mode == 0) go(I0{}, I3{}); else go(I1{}, I3{});
+  } else if (dimu == 2 && NDPN != 3) {
+    if (mode == 0) go(I0{}, I2{}); else go(I1{}, I2{});
+  } else {
+    if (mode == 0) go(I0{}, I0{}); else go(I1{}, I0{});
+  }
   TL_CHECK_LAUNCH();
   return prof_end(P, ev);
 }
