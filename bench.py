@@ -486,7 +486,8 @@ def main():
         variants = {}
         forms = [("fp64_vcycle", "fp64", 1, 1, False), ("fp32_vcycle", "fp32", 1, 1, False),
                  ("bf16_mfma_vcycle", "bf16", 1, 1, False), ("fp16_mfma_vcycle", "fp16", 1, 1, False),
-                 ("fp16_mfma_wcycle_l2", "fp16", 2, 2, False)]
+                 ("fp16_mfma_wcycle_l2", "fp16", 2, 2, False), ("fp32_wcycle_l2", "fp32", 2, 2, False),
+                 ("fp64_wcycle_l2", "fp64", 2, 2, False)]
         if net is not None:
             forms += [("fp64_vcycle_learned", "fp64", 1, 1, True),
                       ("fp16_mfma_wcycle_l2_learned", "fp16", 2, 2, True)]
