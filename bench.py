@@ -135,7 +135,7 @@ def kernel_kinds(dims, vcycle, nlevels, fp64_mfma=False):
     }
 
 
-def bench_batched(dev, steps=2, nstreams=12, chunk=8, cpu_sample=True):
+def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True):
     """Batched uneven 2D solves: 32 Q4 elasticity grids (4 load cases each)
     and 8 Q4 heat grids from workloads.batched_instances(seed=0), FP64 outer
     PCG with FP32 V-cycles to relres 1e-8.  A step = set_density +
@@ -196,7 +196,7 @@ def bench_batched(dev, steps=2, nstreams=12, chunk=8, cpu_sample=True):
         "workload": "32 Q4 elasticity grids (nelx 128..1024, nely 64..512, multiples of 32, "
                     "4 point-load cases, left edge clamped, Emin 1e-9) + 8 Q4 heat grids "
                     "(kmin 1e-3, uniform source, left edge T=0); GRF densities seed 0; "
-                    "FP64 PCG, FP32 GMG V(1,1) (coarsest grid <= 5000 DOF), relres 1e-8",
+                    "FP64 PCG, FP32 GMG V(1,1) (coarsest grid <= 3000 DOF), relres 1e-8",
         "instances": len(ops), "systems": len(its), "dof_total": int(sum(op.ndof for op in ops)),
         "dof_rhs_total": int(dof_rhs),
         "ms_per_step": round(ms_step, 2), "ms_setup": round(ms_setup, 2),

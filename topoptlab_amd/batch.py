@@ -52,7 +52,7 @@ def clear_autotune_table() -> None:
 
 def batch_solve(ops: Sequence[StiffnessOperator], fs: Sequence[torch.Tensor], rtol: float = 1e-8,
                 atol: float = 0.0, maxiter: int = 100000, preconditioner: str = "gmg",
-                nstreams: int = 12, chunk: int = 8, use_graphs: bool = True,
+                nstreams: int = 16, chunk: int = 8, use_graphs: bool = True,
                 x0: Optional[Sequence[torch.Tensor]] = None, raise_on_fail: bool = True):
     """Solve K_i u = f_i for every instance i; ``fs[i]`` holds one (ndof,) or
     several (nrhs, ndof) full-length float64 right-hand sides.  Returns the
@@ -105,7 +105,7 @@ def batch_solve(ops: Sequence[StiffnessOperator], fs: Sequence[torch.Tensor], rt
     return sols, out_info
 
 
-def build_instance(spec: Dict, device, max_coarse_dof: int = 5000):
+def build_instance(spec: Dict, device, max_coarse_dof: int = 3000):
     """StiffnessOperator, density and right-hand sides of one instance of
     ``workloads.batched_instances`` (E0=1, Emin=1e-9, nu=0.3 elasticity;
     k0=1, kmin=1e-3 heat; p=3)."""
