@@ -489,25 +489,25 @@ __global__ void pad_identity_kernel(double* A, int64_t n, int64_t np) {
 // columns j outside K and saves the old column panel of its 64 rows; the
 // first workgroup also saves P^-1.  Launch 2 (64x64 tiles): trailing update
 // A[i][j] -= C[i] R[j], column panel A[i][K] = -C[i] P^-1, pivot block P^-1.
-__device__ inline void gj_invert_pivot(double (*a)[GJB + 1], double (*w)[GJB + 1]) {
-  // in-place Gauss-Jordan inverse of the GJB x GJB LDS block a, BS threads;
-  // step t reads one buffer and writes the other (one barrier per step), and
-  // an even GJB leaves the result in a
-  static_assert(GJB % 2 == 0, "the ping-pong inverse ends in a for even GJB");
+__device__ inline void gj_invert_pivot(double (*a)[GJB + 1]) {
+  // in-place Gauss-Jordan inverse of the GJB x GJB LDS block, BS threads
   for (int t = 0; t < GJB; ++t) {
-    double (*src)[GJB + 1] = (t & 1) ? w : a;
-    double (*dst)[GJB + 1] = (t & 1) ? a : w;
-    const double inv = 1.0 / src[t][t];  // one reciprocal per step
+    double nv[GJB * GJB / BS];
+    const double inv = 1.0 / a[t][t];  // one reciprocal per step
 #pragma unroll
     for (int u = 0; u < GJB * GJB / BS; ++u) {
       const int e = threadIdx.x + u * BS, i = e / GJB, j = e % GJB;
-      const double ait = src[i][t], atj = src[t][j] * inv, aij = src[i][j];
-      double nv;
-      if (i == t && j == t) nv = inv;
-      else if (i == t) nv = atj;
-      else if (j == t) nv = -ait * inv;
-      else nv = aij - ait * atj;
-      dst[i][j] = nv;
+      const double ait = a[i][t], atj = a[t][j] * inv, aij = a[i][j];
+      if (i == t && j == t) nv[u] = inv;
+      else if (i == t) nv[u] = atj;
+      else if (j == t) nv[u] = -ait * inv;
+      else nv[u] = aij - ait * atj;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < GJB * GJB / BS; ++u) {
+      const int e = threadIdx.x + u * BS;
+      a[e / GJB][e % GJB] = nv[u];
     }
     __syncthreads();
   }
@@ -521,7 +521,6 @@ __global__ __launch_bounds__(BS) void gj_panel_kernel(double* A, int64_t np, int
                                                       double* __restrict__ pinv) {
   static_assert(BS == 4 * GJ_COLS && GJB % 4 == 0, "four waves of 64 columns");
   __shared__ double a[GJB][GJB + 1];
-  __shared__ double wbuf[GJB][GJB + 1];
   const int64_t kb = int64_t(k) * GJB;
   const int64_t r0 = int64_t(blockIdx.x) * GJ_COLS;
   for (int e = threadIdx.x; e < GJB * GJB; e += BS) a[e / GJB][e % GJB] = A[(kb + e / GJB) * np + kb + e % GJB];
@@ -536,7 +535,7 @@ __global__ __launch_bounds__(BS) void gj_panel_kernel(double* A, int64_t np, int
 #pragma unroll
   for (int t = 0; t < GJB; ++t) col[t] = active ? A[(kb + t) * np + j] : 0.0;
   __syncthreads();  // every old value read before any row-panel write below
-  gj_invert_pivot(a, wbuf);
+  gj_invert_pivot(a);
   if (blockIdx.x == 0)
     for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv[e] = a[e / GJB][e % GJB];
   if (!active) return;
@@ -559,17 +558,6 @@ __global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, i
   __shared__ double xp[GJB][64 + 1];
   const int64_t kb = int64_t(k) * GJB;
   const int64_t i0 = int64_t(blockIdx.y) * 64, j0 = int64_t(blockIdx.x) * 64;
-  const int tj = threadIdx.x % 16, ti = threadIdx.x / 16;  // 16 x 16 threads, 4x4 each
-  // the thread's 4x4 block of A, requested before the panels are staged so
-  // that its latency overlaps the staging and the FMAs
-  double aold[4][4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u)
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t i = min(i0 + ti * 4 + u, np - 1), j = min(j0 + tj * 4 + v, np - 1);
-      aold[u][v] = A[i * np + j];
-    }
   for (int t = threadIdx.x; t < 64 * GJB; t += BS) {
     const int r = t / GJB, c = t % GJB;
     cp[r][c] = (i0 + r < np) ? colsave[(i0 + r) * GJB + c] : 0.0;
@@ -580,6 +568,7 @@ __global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, i
     xp[r2][c2] = x;
   }
   __syncthreads();
+  const int tj = threadIdx.x % 16, ti = threadIdx.x / 16;  // 16 x 16 threads, 4x4 each
   double acc[4][4] = {};
 #pragma unroll 8
   for (int t = 0; t < GJB; ++t) {
@@ -607,7 +596,7 @@ __global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, i
       } else if (jcol) {
         A[i * np + j] = -acc[u][v];
       } else {
-        A[i * np + j] = aold[u][v] - acc[u][v];
+        A[i * np + j] -= acc[u][v];
       }
     }
   }
