@@ -49,8 +49,11 @@ def _problem(dims, ndpn, seed=0, bc="cantilever"):
     return KE, edof, ndof, fixed, free, x
 
 
+# 3D cases cover one tile column (6x4x5), workgroup ranges that cross tile
+# columns (24x16x20, 40x30x6) and node planes wider than a workgroup in y, so
+# that each plane copy takes several rounds (4x300x3: ny = 301 > 256)
 CASES = [((12, 8, None), 2), ((10, 6, None), 1), ((6, 4, 5), 3), ((5, 4, 3), 1),
-         ((24, 16, 20), 3), ((300, 7, None), 2)]
+         ((24, 16, 20), 3), ((300, 7, None), 2), ((4, 300, 3), 3), ((40, 30, 6), 1)]
 
 
 @pytest.mark.parametrize("dims,ndpn", CASES)
@@ -113,6 +116,36 @@ def test_pcg_gmg_fp64_matches_oracle(gpu, dims, ndpn, nlevels):
     uj, infj = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10,
                         preconditioner="jacobi", maxiter=200000)
     assert rel(uj[op._free_t].cpu().numpy(), ud) < 1e-6
+    op.close()
+
+
+def test_pcg_gmg_fp64_matches_oracle_large_level(gpu):
+    """128x64x64 H8 heat conduction (545k DOF), 4 levels: level 1 has 70,785
+    nodes, above the 65,536 up to which the coarse stencil kernel runs its
+    unrolled small-level form, so this case pins the looped form (the form
+    of the 192x96x96 workload's level 1, there with 3 DOFs per node; the
+    elasticity oracle at this node count takes minutes) against the oracle's
+    V-cycle; levels 2 and 3 take the unrolled form."""
+    import torch
+    from topoptlab_amd.fem import StiffnessOperator
+    dims, ndpn, nlevels = (128, 64, 64), 1, 4
+    say("gmg fp64 large level", dims)
+    KE, edof, ndof, fixed, free, _ = _problem(dims, ndpn)
+    x = workloads.gaussian_random_density(*dims, seed=5, sigma=2.0)
+    K = fem.apply_bc(fem.assemble_K(KE, edof, fem.simp(x), ndof), free).tocsr()
+    b = np.random.default_rng(3).standard_normal(len(free))
+    h = mg.Hierarchy(K, *dims, ndpn, fixed, nlevels)
+    u_ref, info, it_ref = mg.pcg(K, b, h.vcycle, rtol=1e-8)
+    assert info == 0
+    say("oracle iterations", it_ref)
+    op = StiffnessOperator(*dims, KE=KE, fixed=fixed, device=gpu)
+    op.set_density(x)
+    op.setup_mg(nlevels=nlevels, precision="fp64")
+    u, inf = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-8)
+    u = u[op._free_t].cpu().numpy()
+    assert abs(inf["iterations"] - it_ref) <= 1, (inf, it_ref)
+    assert rel(u, u_ref) < 1e-6
+    assert inf["relres_true"] < 2e-8
     op.close()
 
 
