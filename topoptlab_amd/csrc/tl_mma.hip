@@ -30,17 +30,20 @@ constexpr int BS = 256;
 constexpr int NR = 4;  // reduced values per pass
 
 struct MmaWork {
-  DevBuf vec, partials, out;
+  DevBuf vec, partials;
+  // the folded scalars, written by the fold kernel straight into pinned,
+  // coherent host memory: the host reads them after the stream synchronises,
+  // with no copy in between (one per fold, ~90 folds per update)
+  double* hout = nullptr;
   int64_t n = 0;
   int ensure(int64_t nn) {
     if (nn > n) {
       TL_TRY(vec.alloc(sizeof(double) * size_t(nn) * 15));
       n = nn;
     }
-    if (!partials.ptr) {
-      TL_TRY(partials.alloc(sizeof(double) * NR * kMaxRedBlocks));
-      TL_TRY(out.alloc(sizeof(double) * NR));
-    }
+    if (!partials.ptr) TL_TRY(partials.alloc(sizeof(double) * NR * kMaxRedBlocks));
+    if (!hout) TL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hout), sizeof(double) * NR,
+                                    hipHostMallocCoherent));
     return TL_OK;
   }
 };
@@ -273,13 +276,13 @@ extern "C" int tl_mma_update(int64_t n, double* x, const double* xold1, const do
   for (int k = 0; k < 15; ++k) *slots[k] = base + size_t(k) * size_t(n);
   const unsigned nb = unsigned(std::min<int64_t>((n + BS - 1) / BS, 1024));
   double* partials = W.partials.as<double>();
-  double* out = W.out.as<double>();
+  double* out = W.hout;
   double h[NR];
   auto fold = [&](unsigned maxmask) -> int {
     hipLaunchKernelGGL(mma_fold_kernel, dim3(1), dim3(BS), 0, s, partials, nb, maxmask, out);
     TL_CHECK_LAUNCH();
-    TL_HIP(hipMemcpyAsync(h, out, sizeof(h), hipMemcpyDeviceToHost, s));
     TL_HIP(hipStreamSynchronize(s));
+    for (int k = 0; k < NR; ++k) h[k] = static_cast<volatile double*>(out)[k];
     return TL_OK;
   };
   // approximation and the subproblem's start point
