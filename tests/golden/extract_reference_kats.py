@@ -7,7 +7,7 @@ known-answer data of
                                  optimizers, heat)
   tests/test_linsolver.py:10-28  the 3x3 linear-system known answer
 and stores it as JSON.  Run once in a container that has the reference at
-/root/reference; the JSON is committed, the GPU box never needs the reference.
+/root/reference; the JSON is committed, so the tests never need the reference.
 """
 from __future__ import annotations
 

@@ -8,7 +8,7 @@ hierarchy, to the shape of the displacement there.  Inside PCG it provides
 the coarse-grid part of the initial guess: the level-1 field is prolongated
 to the fine grid and scaled by the energy-optimal factor
 alpha = (f.x0)/(x0.K x0), so the guess is never worse than zero in the
-energy norm.  The plain GMG-PCG stays the judge: the solve runs to the same
+energy norm.  The plain GMG-PCG stays the arbiter: the solve runs to the same
 residual tolerance from the learned guess, and falls back to the zero guess
 (the plain method) when the network is unavailable or the scaled guess is
 degenerate.
