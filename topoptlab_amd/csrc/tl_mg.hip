@@ -325,10 +325,13 @@ __global__ void jacobi_zero_kernel(const TB* __restrict__ b, const T* __restrict
 // coarse b = P^T r (masked).  The 27 (2D: 9) fine neighbours are read from
 // clamped positions, those outside the grid with weight 0, so every load is
 // issued before the first sum (a loop that stepped over them waited for each
-// load in turn); the summation order is unchanged.
+// load in turn); the summation order is unchanged.  With xc set, the coarse
+// level's first smoothing sweep from the zero guess, x = omega D^-1 b, is
+// written in the same pass (the coarse cycle then starts from it).
 template <int NDPN, typename T, bool D3>
 __global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
                                 const uint8_t* __restrict__ maskc, T* __restrict__ bc,
+                                const T* __restrict__ dinvc, T omega, T* __restrict__ xc,
                                 const int* done) {
   if (*done) return;
   const int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
@@ -356,7 +359,11 @@ __global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
       }
   const uint8_t mk = maskc[n];
 #pragma unroll
-  for (int d = 0; d < NDPN; ++d) bc[n * NDPN + d] = ((mk >> d) & 1) ? T(0) : acc[d];
+  for (int d = 0; d < NDPN; ++d) {
+    const T v = ((mk >> d) & 1) ? T(0) : acc[d];
+    bc[n * NDPN + d] = v;
+    if (xc) xc[n * NDPN + d] = zero_guess_sweep<T, T>(omega, dinvc[n * NDPN + d], v);
+  }
 }
 
 // fine x += P xc (masked).  The up to 8 coarse nodes are read from clamped
@@ -397,14 +404,15 @@ __global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
 
 template <int NDPN, typename T>
 int launch_restrict(tl_problem* P, const Grid& gf, const Grid& gc, const T* rf,
-                    const uint8_t* maskc, T* bc, const int* done) {
+                    const uint8_t* maskc, T* bc, const T* dinvc, T omega, T* xc,
+                    const int* done) {
   const dim3 grid(unsigned((gc.nnodes + BS - 1) / BS));
   if (gf.dim == 3)
     hipLaunchKernelGGL((restrict_kernel<NDPN, T, true>), grid, dim3(BS), 0, P->stream, to_d(gf),
-                       to_d(gc), rf, maskc, bc, done);
+                       to_d(gc), rf, maskc, bc, dinvc, omega, xc, done);
   else
     hipLaunchKernelGGL((restrict_kernel<NDPN, T, false>), grid, dim3(BS), 0, P->stream, to_d(gf),
-                       to_d(gc), rf, maskc, bc, done);
+                       to_d(gc), rf, maskc, bc, dinvc, omega, xc, done);
   TL_CHECK_LAUNCH();
   return TL_OK;
 }
@@ -676,6 +684,22 @@ T* vp(const DevBuf& b) { return static_cast<T*>(b.ptr); }
 // Host-side guard: every buffer a V-cycle kernel dereferences must exist.
 #define TL_NONNULL(p) TL_ARG((p) != nullptr, "internal: unallocated multigrid buffer " #p)
 
+// Restriction of a residual into level l+1.  Below it a stencil level starts
+// its cycle from x = omega D^-1 b, written by the same kernel; the coarsest
+// level (dense solve) takes b alone.
+template <int NDPN, typename T>
+int restrict_to(tl_problem* P, int l, const Grid& gf, const T* rf, const int* done) {
+  MgState& M = P->mg;
+  Level& C = M.lv[l + 1];
+  const bool seed = l + 1 < M.nlevels - 1;
+  const T* dinvc = std::is_same<T, double>::value ? static_cast<const T*>(C.dinv64.ptr)
+                                                   : static_cast<const T*>(C.dinv32.ptr);
+  if (seed) { TL_NONNULL(dinvc); TL_NONNULL(C.vx.ptr); }
+  return launch_restrict<NDPN, T>(P, gf, C.g, rf, C.mask.as<uint8_t>(), vp<T>(C.vb),
+                                  seed ? dinvc : nullptr, T(M.omega), seed ? vp<T>(C.vx) : nullptr,
+                                  done);
+}
+
 template <int NDPN, typename T>
 int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const int* done) {
   Level& L = P->mg.lv[l];
@@ -717,7 +741,8 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
   return prof_end(P, ev);
 }
 
-// Solve A_l x_l = b_l approximately (x_l := V-cycle(b_l), zero initial guess).
+// Solve A_l x_l = b_l approximately (x_l := V-cycle(b_l) from the zero guess; the
+// caller's restriction has already written the first sweep into x_l).
 template <int NDPN, typename T>
 int vcycle_level(tl_problem* P, int l, const int* done) {
   MgState& M = P->mg;
@@ -737,17 +762,13 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
   T* x = vp<T>(L.vx);
   T* t = vp<T>(L.vt);
   const T* b = vp<T>(L.vb);
-  const T* dinv = std::is_same<T, double>::value ? static_cast<const T*>(L.dinv64.ptr)
-                                                  : static_cast<const T*>(L.dinv32.ptr);
   // gamma > 1 (W-cycle): the level's cycle repeats from the current x; each
   // repetition smooths, corrects from a fresh coarse cycle and smooths again
   const int gam = l >= M.gamma_from ? M.gamma : 1;
   for (int g = 0; g < gam; ++g) {
-    if (g == 0) {
-      hipLaunchKernelGGL((jacobi_zero_kernel<T, T>), dim3(gsz(n)), dim3(BS), 0, P->stream, b, dinv,
-                         T(M.omega), x, n, done);
-      TL_CHECK_LAUNCH();
-    } else {
+    // g == 0: the first sweep from the zero guess, x = omega D^-1 b, was
+    // written by the restriction that produced b (restrict_to)
+    if (g > 0) {
       TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
       std::swap(L.vx.ptr, L.vt.ptr);
       x = vp<T>(L.vx); t = vp<T>(L.vt);
@@ -759,8 +780,7 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
     }
     TL_TRY((st_apply<NDPN, T>(P, l, 0, x, b, vp<T>(L.vr), done)));
     Level& C = M.lv[l + 1];
-    TL_TRY((launch_restrict<NDPN, T>(P, L.g, C.g, vp<T>(L.vr), C.mask.as<uint8_t>(), vp<T>(C.vb),
-                                      done)));
+    TL_TRY((restrict_to<NDPN, T>(P, l, L.g, vp<T>(L.vr), done)));
     TL_TRY((vcycle_level<NDPN, T>(P, l + 1, done)));
     TL_TRY((launch_prolong<NDPN, T>(P, L.g, C.g, vp<T>(C.vx), L.mask.as<uint8_t>(), x, done)));
     for (int s = 0; s < M.nu2; ++s) {
@@ -804,8 +824,7 @@ int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done, bool
   }
   TL_TRY(fine_apply_ex(P, x, vp<T>(L.vr), prec, KU_RESID, r, true, nullptr, 0.0, nullptr, nullptr,
                        nullptr, nullptr, 0, done, false));
-  TL_TRY((launch_restrict<NDPN, T>(P, L.g, C.g, vp<T>(L.vr), C.mask.as<uint8_t>(), vp<T>(C.vb),
-                                    done)));
+  TL_TRY((restrict_to<NDPN, T>(P, 0, L.g, vp<T>(L.vr), done)));
   TL_TRY((vcycle_level<NDPN, T>(P, 1, done)));
   // the fine grid's Dirichlet mask is the operator's (level 0 has no copy)
   TL_TRY((launch_prolong<NDPN, T>(P, L.g, C.g, vp<T>(C.vx), P->op.mask.as<uint8_t>(), x, done)));
