@@ -225,12 +225,50 @@ __global__ void st_finish_kernel(GridD g, int ndpn, const double* __restrict__ s
 // neighbours.  Workgroups take contiguous node ranges per XCD so that the
 // neighbours' blocks (up to one node plane back) are re-read from the XCD's
 // own L2.
+// (P xc) at fine node (ix, iy, iz): the up to 8 (2D: 4) coarse nodes are read
+// from clamped positions, all loads before the first sum, the unused ones with
+// weight 0.  Shared by the prolongation and the fused post-smoothing sweep so
+// that both add the same value.
+template <int NDPN, typename T, bool D3>
+__device__ __forceinline__ void prolong_at(const GridD& gc, const T* __restrict__ xc, int ix,
+                                           int iy, int iz, T (&acc)[NDPN]) {
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
+  const int x0 = ix >> 1, x1 = (ix + 1) >> 1, y0 = iy >> 1, y1 = (iy + 1) >> 1;
+  const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
+  const T wx = (ix & 1) ? T(0.5) : T(1), wy = (iy & 1) ? T(0.5) : T(1), wz = (iz & 1) ? T(0.5) : T(1);
+#pragma unroll
+  for (int a = 0; a < (D3 ? 2 : 1); ++a)
+#pragma unroll
+    for (int bx = 0; bx < 2; ++bx)
+#pragma unroll
+      for (int by = 0; by < 2; ++by) {
+        const int cz = z0 + a, cx = x0 + bx, cy = y0 + by;
+        const bool ok = cz <= z1 && cx <= x1 && cy <= y1;
+        const int64_t ci = node_id(gc, min(cx, gc.nx - 1), min(cy, gc.ny - 1), min(cz, gc.nz - 1));
+        const T w = ok ? wx * wy * wz : T(0);
+#pragma unroll
+        for (int d = 0; d < NDPN; ++d) acc[d] += w * xc[ci * NDPN + d];
+      }
+}
+
+// Coarse correction applied on the fly by the post-smoothing sweep that
+// follows a prolongation: x'(m) = x(m) + (P xc)(m) on the free DOFs of m.
+template <typename T>
+struct CoarseCorr {
+  GridD gc;
+  const T* xc = nullptr;
+  const uint8_t* mask = nullptr;  // the fine level's
+};
+
 // One 27-point (2D: 9-point) term A(n, m) x_m, m = n + (rx, ry, rz).  DIM is
-// the grid dimension when known at compile time (0: read it from g).
-template <int NDPN, typename T, int DIM>
+// the grid dimension when known at compile time (0: read it from g).  PRO: x_m
+// is x'(m) of the coarse correction cc (DIM 2 or 3 only).
+template <int NDPN, typename T, int DIM, bool PRO = false>
 __device__ __forceinline__ void st_term(const GridD& g, const T* __restrict__ sth,
                                         const T* __restrict__ x, int64_t n, int ix, int iy,
-                                        int iz, int rx, int ry, int rz, T (&acc)[NDPN]) {
+                                        int iz, int rx, int ry, int rz, T (&acc)[NDPN],
+                                        const CoarseCorr<T>& cc = CoarseCorr<T>{}) {
   const int dim = DIM ? DIM : g.dim;
   const int kc = center_k(dim);
   const int k = kidx(dim, rx, ry, rz);
@@ -241,6 +279,15 @@ __device__ __forceinline__ void st_term(const GridD& g, const T* __restrict__ st
   T xm[NDPN];
 #pragma unroll
   for (int c = 0; c < NDPN; ++c) xm[c] = x[m * NDPN + c];
+  if constexpr (PRO) {
+    static_assert(DIM == 2 || DIM == 3, "the fused correction needs a compile-time dimension");
+    T pc[NDPN];
+    prolong_at<NDPN, T, DIM == 3>(cc.gc, cc.xc, mx, my, mz, pc);
+    const uint8_t mkm = cc.mask[m];
+#pragma unroll
+    for (int c = 0; c < NDPN; ++c)
+      if (!((mkm >> c) & 1)) xm[c] += pc[c];
+  }
   if (k >= kc) {  // forward block of node n (zero towards missing neighbours)
     const T* h = sth + int64_t(k - kc) * NDPN * NDPN * g.nnodes + n;
 #pragma unroll
@@ -266,13 +313,19 @@ __device__ __forceinline__ void st_term(const GridD& g, const T* __restrict__ st
 // forward/backward choice is resolved at compile time and the loads of all
 // terms are in flight together (small levels, whose few waves cannot hide a
 // chain of dependent loads).  Same terms in the same order either way.
-template <int NDPN, typename T, int MODE, int DIMU>
+// PRO (MODE 1, DIMU 2 or 3): x is the iterate before the coarse correction cc,
+// which is added on the fly (prolongation and first post-smoothing sweep in
+// one pass, with the values the two separate passes give).
+template <int NDPN, typename T, int MODE, int DIMU, bool PRO = false>
 __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restrict__ sth,
                                                       const T* __restrict__ x,
                                                       const T* __restrict__ b,
                                                       const T* __restrict__ dinv, T omega,
                                                       const uint8_t* __restrict__ mask,
-                                                      T* __restrict__ y, const int* done) {
+                                                      T* __restrict__ y, const int* done,
+                                                      CoarseCorr<T> cc) {
+  static_assert(!PRO || (MODE == 1 && DIMU != 0 && NDPN != 3),
+                "fused correction: unrolled Jacobi sweep, 1 or 2 DOFs per node");
   if (*done) return;
   const unsigned nb = gridDim.x, hw = blockIdx.x;
   const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
@@ -299,15 +352,19 @@ __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restri
       for (int rx = -1; rx <= 1; ++rx)
 #pragma unroll
         for (int ry = -1; ry <= 1; ++ry)
-          st_term<NDPN, T, DIMU>(g, sth, x, n, ix, iy, iz, rx, ry, rz, acc);
+          st_term<NDPN, T, DIMU, PRO>(g, sth, x, n, ix, iy, iz, rx, ry, rz, acc, cc);
   }
   const uint8_t mk = mask[n];
+  T pn[NDPN];
+  if constexpr (PRO) prolong_at<NDPN, T, DIMU == 3>(cc.gc, cc.xc, ix, iy, iz, pn);
 #pragma unroll
   for (int d = 0; d < NDPN; ++d) {
     const int64_t i = n * NDPN + d;
+    T xi = x[i];
+    if constexpr (PRO) xi += pn[d];  // only read on free DOFs (fixed ones give 0)
     T v;
     if (MODE == 0) v = b[i] - acc[d];
-    else v = x[i] + omega * dinv[i] * (b[i] - acc[d]);
+    else v = xi + omega * dinv[i] * (b[i] - acc[d]);
     y[i] = ((mk >> d) & 1) ? T(0) : v;
   }
 }
@@ -328,7 +385,7 @@ __global__ void jacobi_zero_kernel(const TB* __restrict__ b, const T* __restrict
 // load in turn); the summation order is unchanged.  With xc set, the coarse
 // level's first smoothing sweep from the zero guess, x = omega D^-1 b, is
 // written in the same pass (the coarse cycle then starts from it).
-template <int NDPN, typename T, bool D3>
+template <int NDPN, typename T, bool D3, bool SEED>
 __global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
                                 const uint8_t* __restrict__ maskc, T* __restrict__ bc,
                                 const T* __restrict__ dinvc, T omega, T* __restrict__ xc,
@@ -362,12 +419,11 @@ __global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
   for (int d = 0; d < NDPN; ++d) {
     const T v = ((mk >> d) & 1) ? T(0) : acc[d];
     bc[n * NDPN + d] = v;
-    if (xc) xc[n * NDPN + d] = zero_guess_sweep<T, T>(omega, dinvc[n * NDPN + d], v);
+    if constexpr (SEED) xc[n * NDPN + d] = zero_guess_sweep<T, T>(omega, dinvc[n * NDPN + d], v);
   }
 }
 
-// fine x += P xc (masked).  The up to 8 coarse nodes are read from clamped
-// positions, all loads before the first sum, the unused ones with weight 0.
+// fine x += P xc (masked)
 template <int NDPN, typename T, bool D3>
 __global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
                                    const uint8_t* __restrict__ maskf, T* __restrict__ xf,
@@ -378,24 +434,7 @@ __global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
   int ix, iy, iz;
   node_xyz(gf, n, ix, iy, iz);
   T acc[NDPN];
-#pragma unroll
-  for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
-  const int x0 = ix >> 1, x1 = (ix + 1) >> 1, y0 = iy >> 1, y1 = (iy + 1) >> 1;
-  const int z0 = iz >> 1, z1 = (iz + 1) >> 1;
-  const T wx = (ix & 1) ? T(0.5) : T(1), wy = (iy & 1) ? T(0.5) : T(1), wz = (iz & 1) ? T(0.5) : T(1);
-#pragma unroll
-  for (int a = 0; a < (D3 ? 2 : 1); ++a)
-#pragma unroll
-    for (int bx = 0; bx < 2; ++bx)
-#pragma unroll
-      for (int by = 0; by < 2; ++by) {
-        const int cz = z0 + a, cx = x0 + bx, cy = y0 + by;
-        const bool ok = cz <= z1 && cx <= x1 && cy <= y1;
-        const int64_t ci = node_id(gc, min(cx, gc.nx - 1), min(cy, gc.ny - 1), min(cz, gc.nz - 1));
-        const T w = ok ? wx * wy * wz : T(0);
-#pragma unroll
-        for (int d = 0; d < NDPN; ++d) acc[d] += w * xc[ci * NDPN + d];
-      }
+  prolong_at<NDPN, T, D3>(gc, xc, ix, iy, iz, acc);
   const uint8_t mk = maskf[n];
 #pragma unroll
   for (int d = 0; d < NDPN; ++d)
@@ -407,12 +446,15 @@ int launch_restrict(tl_problem* P, const Grid& gf, const Grid& gc, const T* rf,
                     const uint8_t* maskc, T* bc, const T* dinvc, T omega, T* xc,
                     const int* done) {
   const dim3 grid(unsigned((gc.nnodes + BS - 1) / BS));
-  if (gf.dim == 3)
-    hipLaunchKernelGGL((restrict_kernel<NDPN, T, true>), grid, dim3(BS), 0, P->stream, to_d(gf),
-                       to_d(gc), rf, maskc, bc, dinvc, omega, xc, done);
-  else
-    hipLaunchKernelGGL((restrict_kernel<NDPN, T, false>), grid, dim3(BS), 0, P->stream, to_d(gf),
-                       to_d(gc), rf, maskc, bc, dinvc, omega, xc, done);
+  auto go = [&](auto d3_c, auto seed_c) {
+    hipLaunchKernelGGL((restrict_kernel<NDPN, T, decltype(d3_c)::value, decltype(seed_c)::value>),
+                       grid, dim3(BS), 0, P->stream, to_d(gf), to_d(gc), rf, maskc, bc, dinvc,
+                       omega, xc, done);
+  };
+  using F = std::false_type;
+  using Tr = std::true_type;
+  if (gf.dim == 3) { if (xc) go(Tr{}, Tr{}); else go(Tr{}, F{}); }
+  else { if (xc) go(F{}, Tr{}); else go(F{}, F{}); }
   TL_CHECK_LAUNCH();
   return TL_OK;
 }
@@ -700,6 +742,17 @@ int restrict_to(tl_problem* P, int l, const Grid& gf, const T* rf, const int* do
                                   done);
 }
 
+// Levels of up to 64k nodes (a few hundred workgroups) take the unrolled
+// stencil form (2 or 3: the dimension; 0: the looped form).  NDPN 2 only
+// occurs in 2D and NDPN 3 in 3D for the unrolled forms.
+template <int NDPN>
+int st_dimu(const Grid& g) {
+  if (g.nnodes > 65536) return 0;
+  if (g.dim == 3 && NDPN != 2) return 3;
+  if (g.dim == 2 && NDPN != 3) return 2;
+  return 0;
+}
+
 template <int NDPN, typename T>
 int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const int* done) {
   Level& L = P->mg.lv[l];
@@ -715,27 +768,76 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
   ProfEvent ev{};
   ev.kind = -1;
   if (l == 1) TL_TRY(prof_begin(P, 3 + mode, ev));
-  // levels of up to 64k nodes (a few hundred workgroups) take the unrolled
-  // form; NDPN 2 only occurs in 2D and NDPN 3 in 3D for the unrolled forms
-  const int dimu = L.g.nnodes <= 65536 ? L.g.dim : 0;
+  const int dimu = st_dimu<NDPN>(L.g);
   const GridD gd = to_d(L.g);
   const T om = T(P->mg.omega);
   const uint8_t* mk = L.mask.as<uint8_t>();
   auto go = [&](auto mode_c, auto dim_c) {
     constexpr int MODE_ = decltype(mode_c)::value, DIM_ = decltype(dim_c)::value;
     hipLaunchKernelGGL((st_apply_kernel<NDPN, T, MODE_, DIM_>), dim3(nb), dim3(BS), 0, P->stream,
-                       gd, st, x, b, dinv, om, mk, y, done);
+                       gd, st, x, b, dinv, om, mk, y, done, CoarseCorr<T>{});
   };
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
   using I3 = std::integral_constant<int, 3>;
-  if (dimu == 3 && NDPN != 2) {
-    if (mode == 0) go(I0{}, I3{}); else go(I1{}, I3{});
-  } else if (dimu == 2 && NDPN != 3) {
-    if (mode == 0) go(I0{}, I2{}); else go(I1{}, I2{});
+  if (dimu == 3) {
+    if constexpr (NDPN != 2) { if (mode == 0) go(I0{}, I3{}); else go(I1{}, I3{}); }
+  } else if (dimu == 2) {
+    if constexpr (NDPN != 3) { if (mode == 0) go(I0{}, I2{}); else go(I1{}, I2{}); }
   } else {
     if (mode == 0) go(I0{}, I0{}); else go(I1{}, I0{});
+  }
+  TL_CHECK_LAUNCH();
+  return prof_end(P, ev);
+}
+
+// Whether the prolongation into level l is fused with its first post-smoothing
+// sweep: on the unrolled levels with 1 or 2 DOFs per node.  With 3 (3D
+// elasticity) the 27 neighbours' extra coarse gathers exceed the registers
+// (2 KB of scratch per lane in FP64), so the two passes stay separate there.
+template <int NDPN>
+bool fuse_correction(const Grid& g) {
+  return NDPN != 3 && st_dimu<NDPN>(g) != 0;
+}
+
+// y = x' + omega D^-1 (b - A x'), x' = x + P xc on the free DOFs: the
+// prolongation from level l+1 and the first post-smoothing sweep of level l
+// in one pass (fuse_correction levels only).
+template <int NDPN, typename T>
+int st_smooth_corrected(tl_problem* P, int l, const T* x, const T* xc, const T* b, T* y,
+                        const int* done) {
+  Level& L = P->mg.lv[l];
+  TL_ARG(l >= 1 && l + 1 < P->mg.nlevels, "internal: stencil level out of range");
+  const int dimu = st_dimu<NDPN>(L.g);
+  TL_ARG(fuse_correction<NDPN>(L.g), "internal: fused correction on a level without it");
+  TL_NONNULL(L.sth64.ptr); TL_NONNULL(L.st32.ptr); TL_NONNULL(L.dinv64.ptr);
+  TL_NONNULL(L.dinv32.ptr); TL_NONNULL(L.mask.ptr); TL_NONNULL(x); TL_NONNULL(xc);
+  TL_NONNULL(b); TL_NONNULL(y);
+  const T* st = std::is_same<T, double>::value ? static_cast<const T*>(L.sth64.ptr)
+                                                : static_cast<const T*>(L.st32.ptr);
+  const T* dinv = std::is_same<T, double>::value ? static_cast<const T*>(L.dinv64.ptr)
+                                                  : static_cast<const T*>(L.dinv32.ptr);
+  const unsigned nb = unsigned((L.g.nnodes + BS - 1) / BS);
+  ProfEvent ev{};
+  ev.kind = -1;
+  if (l == 1) TL_TRY(prof_begin(P, 4, ev));
+  CoarseCorr<T> cc;
+  cc.gc = to_d(P->mg.lv[l + 1].g);
+  cc.xc = xc;
+  cc.mask = L.mask.as<uint8_t>();
+  const GridD gd = to_d(L.g);
+  const T om = T(P->mg.omega);
+  if constexpr (NDPN == 1) {
+    if (dimu == 3)
+      hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1, 3, true>), dim3(nb), dim3(BS), 0, P->stream,
+                         gd, st, x, b, dinv, om, cc.mask, y, done, cc);
+    else
+      hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1, 2, true>), dim3(nb), dim3(BS), 0, P->stream,
+                         gd, st, x, b, dinv, om, cc.mask, y, done, cc);
+  } else if constexpr (NDPN == 2) {
+    hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1, 2, true>), dim3(nb), dim3(BS), 0, P->stream,
+                       gd, st, x, b, dinv, om, cc.mask, y, done, cc);
   }
   TL_CHECK_LAUNCH();
   return prof_end(P, ev);
@@ -782,8 +884,18 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
     Level& C = M.lv[l + 1];
     TL_TRY((restrict_to<NDPN, T>(P, l, L.g, vp<T>(L.vr), done)));
     TL_TRY((vcycle_level<NDPN, T>(P, l + 1, done)));
-    TL_TRY((launch_prolong<NDPN, T>(P, L.g, C.g, vp<T>(C.vx), L.mask.as<uint8_t>(), x, done)));
-    for (int s = 0; s < M.nu2; ++s) {
+    // coarse correction and post-smoothing; on the fuse_correction levels the
+    // correction is added inside the first sweep
+    int s0 = 0;
+    if (fuse_correction<NDPN>(L.g)) {
+      TL_TRY((st_smooth_corrected<NDPN, T>(P, l, x, vp<T>(C.vx), b, t, done)));
+      std::swap(L.vx.ptr, L.vt.ptr);
+      x = vp<T>(L.vx); t = vp<T>(L.vt);
+      s0 = 1;
+    } else {
+      TL_TRY((launch_prolong<NDPN, T>(P, L.g, C.g, vp<T>(C.vx), L.mask.as<uint8_t>(), x, done)));
+    }
+    for (int s = s0; s < M.nu2; ++s) {
       TL_TRY((st_apply<NDPN, T>(P, l, 1, x, b, t, done)));
       std::swap(L.vx.ptr, L.vt.ptr);
       x = vp<T>(L.vx); t = vp<T>(L.vt);
