@@ -313,7 +313,7 @@ __device__ __forceinline__ void st_term(const GridD& g, const T* __restrict__ st
 // forward/backward choice is resolved at compile time and the loads of all
 // terms are in flight together (small levels, whose few waves cannot hide a
 // chain of dependent loads).  Same terms in the same order either way.
-// PRO (MODE 1, DIMU 2 or 3): x is the iterate before the coarse correction cc,
+// PRO (MODE 1, DIMU 2): x is the iterate before the coarse correction cc,
 // which is added on the fly (prolongation and first post-smoothing sweep in
 // one pass, with the values the two separate passes give).
 template <int NDPN, typename T, int MODE, int DIMU, bool PRO = false>
@@ -324,8 +324,7 @@ __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restri
                                                       const uint8_t* __restrict__ mask,
                                                       T* __restrict__ y, const int* done,
                                                       CoarseCorr<T> cc) {
-  static_assert(!PRO || (MODE == 1 && DIMU != 0 && NDPN != 3),
-                "fused correction: unrolled Jacobi sweep, 1 or 2 DOFs per node");
+  static_assert(!PRO || (MODE == 1 && DIMU == 2), "fused correction: unrolled 2D Jacobi sweep");
   if (*done) return;
   const unsigned nb = gridDim.x, hw = blockIdx.x;
   const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
@@ -793,12 +792,13 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
 }
 
 // Whether the prolongation into level l is fused with its first post-smoothing
-// sweep: on the unrolled levels with 1 or 2 DOFs per node.  With 3 (3D
-// elasticity) the 27 neighbours' extra coarse gathers exceed the registers
-// (2 KB of scratch per lane in FP64), so the two passes stay separate there.
+// sweep: on the small (unrolled-form) 2D levels.  In 3D the 27 neighbours'
+// extra coarse gathers exceed the registers (3 DOFs per node: 2 KB of scratch
+// per lane in FP64; 1 DOF: 256 VGPRs, one wave per SIMD), so the two passes
+// stay separate there.
 template <int NDPN>
 bool fuse_correction(const Grid& g) {
-  return NDPN != 3 && st_dimu<NDPN>(g) != 0;
+  return NDPN != 3 && st_dimu<NDPN>(g) == 2;
 }
 
 // y = x' + omega D^-1 (b - A x'), x' = x + P xc on the free DOFs: the
@@ -828,17 +828,10 @@ int st_smooth_corrected(tl_problem* P, int l, const T* x, const T* xc, const T* 
   cc.mask = L.mask.as<uint8_t>();
   const GridD gd = to_d(L.g);
   const T om = T(P->mg.omega);
-  if constexpr (NDPN == 1) {
-    if (dimu == 3)
-      hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1, 3, true>), dim3(nb), dim3(BS), 0, P->stream,
-                         gd, st, x, b, dinv, om, cc.mask, y, done, cc);
-    else
-      hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1, 2, true>), dim3(nb), dim3(BS), 0, P->stream,
-                         gd, st, x, b, dinv, om, cc.mask, y, done, cc);
-  } else if constexpr (NDPN == 2) {
+  (void)dimu;  // 2: fuse_correction levels are 2D
+  if constexpr (NDPN != 3)
     hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1, 2, true>), dim3(nb), dim3(BS), 0, P->stream,
                        gd, st, x, b, dinv, om, cc.mask, y, done, cc);
-  }
   TL_CHECK_LAUNCH();
   return prof_end(P, ev);
 }
