@@ -129,16 +129,21 @@ __global__ __launch_bounds__(BS) void st_from_fine_kernel(GridD g1, const double
 // ---------------------------------------------------------------------------
 // stencil Galerkin: level l (fine) -> level l+1 (coarse)
 // ---------------------------------------------------------------------------
-// One thread per (coarse node, d, c): every fine stencil value of the 27
-// fine nodes around 2n is read once and added to the 1..8 coarse offsets r
-// it feeds (|dx + sx - 2 rx| <= 1 per axis).  Each coarse entry receives its
-// terms in (fine node, fine offset) order, as in the plain triple product.
+// One thread per (coarse node, d, c).  A_c(n, n+r) = sum over the fine nodes
+// i = 2n + dv around n and their stencil offsets sv of w(dv) A_f(i, i+sv)
+// w(dv + sv - 2r), with the 1D transfer weight w(0) = 1, w(+-1) = 1/2 (else 0)
+// per axis.  The weights factor per axis, so each fine node's block of 27 (2D:
+// 9) values is contracted along z, y and x with 3x3 weight tables (243 instead
+// of 729 multiply-adds, no branches on the offsets).
+__device__ __forceinline__ double tw1(int e) { return e == 0 ? 1.0 : ((e == 1 || e == -1) ? 0.5 : 0.0); }
+
 template <bool D3>
 __global__ __launch_bounds__(BS) void st_coarsen_kernel(GridD gf, GridD gc, int ndpn,
                                                         const double* __restrict__ stf,
                                                         const uint8_t* __restrict__ maskc,
                                                         double* __restrict__ stc) {
   constexpr int NK = D3 ? 27 : 9;
+  constexpr int NZ = D3 ? 3 : 1;
   const int64_t tot = gc.nnodes * ndpn * ndpn;
   for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
        t += int64_t(gridDim.x) * BS) {
@@ -150,27 +155,60 @@ __global__ __launch_bounds__(BS) void st_coarsen_kernel(GridD gf, GridD gc, int 
     double acc[NK];
 #pragma unroll
     for (int k = 0; k < NK; ++k) acc[k] = 0.0;
-    // fine neighbour, fine offset and coarse offset all unrolled: only the
-    // valid (|dx + sx - 2 rx| <= 1 per axis) multiply-adds are emitted
-#pragma unroll
+#pragma unroll 1
     for (int dp = 0; dp < NK; ++dp) {
-          const int dz = D3 ? dp / 9 - 1 : 0, dx = (dp / 3) % 3 - 1, dy = dp % 3 - 1;
-          const int fx = 2 * ix + dx, fy = 2 * iy + dy, fz = D3 ? 2 * iz + dz : 0;
-          if (!node_in(gf, fx, fy, fz)) continue;
-          const double wi = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
-          const int64_t fi = node_id(gf, fx, fy, fz);
+      const int dz = D3 ? dp / 9 - 1 : 0, dx = (dp / 3) % 3 - 1, dy = dp % 3 - 1;
+      const int fx = 2 * ix + dx, fy = 2 * iy + dy, fz = D3 ? 2 * iz + dz : 0;
+      if (!node_in(gf, fx, fy, fz)) continue;
+      const double wi = (dx ? 0.5 : 1.0) * (dy ? 0.5 : 1.0) * (dz ? 0.5 : 1.0);
+      const int64_t fi = node_id(gf, fx, fy, fz);
+      // v[sz][sx][sy]: the fine node's block (d, c) for offset (sx, sy, sz)
+      double v[NZ][3][3];
 #pragma unroll
-          for (int ks = 0; ks < NK; ++ks) {
-            const int sz = D3 ? ks / 9 - 1 : 0, sx = (ks / 3) % 3 - 1, sy = ks % 3 - 1;
-            const double v = stf[int64_t((ks * ndpn + d) * ndpn + c) * gf.nnodes + fi];
+      for (int ks = 0; ks < NK; ++ks)
+        v[D3 ? ks / 9 : 0][(ks / 3) % 3][ks % 3] =
+            stf[int64_t((ks * ndpn + d) * ndpn + c) * gf.nnodes + fi];
+      // contract z: zc[rz][sx][sy] = sum_sz w(dz + sz - 2 rz) v[sz][sx][sy]
+      double zc[NZ][3][3];
 #pragma unroll
-            for (int k = 0; k < NK; ++k) {
-              const int rz = D3 ? k / 9 - 1 : 0, rx = (k / 3) % 3 - 1, ry = k % 3 - 1;
-              const int ez = dz + sz - 2 * rz, ex = dx + sx - 2 * rx, ey = dy + sy - 2 * ry;
-              if (ez < -1 || ez > 1 || ex < -1 || ex > 1 || ey < -1 || ey > 1) continue;
-              const double wj = (ex ? 0.5 : 1.0) * (ey ? 0.5 : 1.0) * (ez ? 0.5 : 1.0);
-              acc[k] += wi * wj * v;
+      for (int rz = 0; rz < NZ; ++rz)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            if constexpr (D3) {
+              double sum = 0.0;
+#pragma unroll
+              for (int sz = 0; sz < 3; ++sz) sum += tw1(dz + (sz - 1) - 2 * (rz - 1)) * v[sz][a][b];
+              zc[rz][a][b] = sum;
+            } else {
+              zc[rz][a][b] = v[0][a][b];
             }
+          }
+      // contract y: yc[rz][sx][ry]
+      double yc[NZ][3][3];
+#pragma unroll
+      for (int rz = 0; rz < NZ; ++rz)
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int ry = 0; ry < 3; ++ry) {
+            double sum = 0.0;
+#pragma unroll
+            for (int sy = 0; sy < 3; ++sy) sum += tw1(dy + (sy - 1) - 2 * (ry - 1)) * zc[rz][a][sy];
+            yc[rz][a][ry] = sum;
+          }
+      // contract x and accumulate: acc[rz][rx][ry] += wi * sum_sx w(dx + sx - 2 rx) yc
+#pragma unroll
+      for (int rz = 0; rz < NZ; ++rz)
+#pragma unroll
+        for (int rx = 0; rx < 3; ++rx)
+#pragma unroll
+          for (int ry = 0; ry < 3; ++ry) {
+            double sum = 0.0;
+#pragma unroll
+            for (int sx = 0; sx < 3; ++sx) sum += tw1(dx + (sx - 1) - 2 * (rx - 1)) * yc[rz][sx][ry];
+            acc[(rz * 3 + rx) * 3 + ry] += wi * sum;
           }
     }
     const bool fn = (maskc[n] >> d) & 1;
