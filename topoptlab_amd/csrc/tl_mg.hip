@@ -64,7 +64,7 @@ __device__ inline int kidx(int dim, int rx, int ry, int rz) {
 // ---------------------------------------------------------------------------
 // level-1 stencil straight from the fine densities
 // ---------------------------------------------------------------------------
-// One thread per (node, d, c): the node's (up to) 8 level-1 elements and
+// One thread per (node, d, c), (d, c) = blockIdx.y: the node's (up to) 8 level-1 elements and
 // their 8 children's scales are loaded once and feed all 27 offsets; each
 // entry sums elements in (oz, ox, oy) order and children in order.
 template <bool D3, int NDPN>
@@ -74,12 +74,12 @@ __global__ __launch_bounds__(BS) void st_from_fine_kernel(GridD g1, const double
                                                           const uint8_t* __restrict__ mask1,
                                                           double* __restrict__ st) {
   constexpr int NEL = D3 ? 8 : 4, NCH = D3 ? 8 : 4, ND = NEL * NDPN, NK = D3 ? 27 : 9;
-  const int64_t tot = g1.nnodes * NDPN * NDPN;
-  for (int64_t t = blockIdx.x * int64_t(BS) + threadIdx.x; t < tot;
-       t += int64_t(gridDim.x) * BS) {
-    const int64_t n = t % g1.nnodes;
-    const int dc = int(t / g1.nnodes);
-    const int d = dc / NDPN, c = dc % NDPN;
+  // (d, c) per grid row: uniform over the workgroup, so the element-matrix
+  // entries are read once per wave (scalar loads), not once per lane
+  const int dc = int(blockIdx.y);
+  const int d = dc / NDPN, c = dc % NDPN;
+  for (int64_t n = blockIdx.x * int64_t(BS) + threadIdx.x; n < g1.nnodes;
+       n += int64_t(gridDim.x) * BS) {
     int ix, iy, iz;
     node_xyz(g1, n, ix, iy, iz);
     // child scales of the elements around the node, e = ((oz+1)*2 + (ox+1))*2 + (oy+1)
@@ -1091,9 +1091,9 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
   // level 1 from the fine densities
   {
     Level& l1 = M.lv[1];
-    const int64_t tot = l1.g.nnodes * ndpn * ndpn;
 #define TL_SFF(D3, NDPN)                                                                      \
-  hipLaunchKernelGGL((st_from_fine_kernel<D3, NDPN>), dim3(gsz(tot)), dim3(BS), 0, s,          \
+  hipLaunchKernelGGL((st_from_fine_kernel<D3, NDPN>), dim3(gsz(l1.g.nnodes), NDPN * NDPN),     \
+                     dim3(BS), 0, s,                                                          \
                      to_d(l1.g), P->op.sp64.as<double>(), g.nelx, g.nely, M.G1.as<double>(),   \
                      l1.mask.as<uint8_t>(), l1.st64.as<double>())
     const bool d3 = g.dim == 3;
@@ -1109,15 +1109,13 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
   for (int l = 2; l < L; ++l) {
     Level& lf = M.lv[l - 1];
     Level& lc = M.lv[l];
-    const int64_t tot = lc.g.nnodes * ndpn * ndpn;
+    const dim3 grid(gsz(lc.g.nnodes * ndpn * ndpn));
     if (g.dim == 3)
-      hipLaunchKernelGGL(st_coarsen_kernel<true>, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lf.g),
-                         to_d(lc.g), ndpn, lf.st64.as<double>(), lc.mask.as<uint8_t>(),
-                         lc.st64.as<double>());
+      hipLaunchKernelGGL(st_coarsen_kernel<true>, grid, dim3(BS), 0, s, to_d(lf.g), to_d(lc.g), ndpn,
+                         lf.st64.as<double>(), lc.mask.as<uint8_t>(), lc.st64.as<double>());
     else
-      hipLaunchKernelGGL(st_coarsen_kernel<false>, dim3(gsz(tot)), dim3(BS), 0, s, to_d(lf.g),
-                         to_d(lc.g), ndpn, lf.st64.as<double>(), lc.mask.as<uint8_t>(),
-                         lc.st64.as<double>());
+      hipLaunchKernelGGL(st_coarsen_kernel<false>, grid, dim3(BS), 0, s, to_d(lf.g), to_d(lc.g), ndpn,
+                         lf.st64.as<double>(), lc.mask.as<uint8_t>(), lc.st64.as<double>());
     TL_CHECK_LAUNCH();
   }
   for (int l = 1; l < L; ++l) {
