@@ -78,7 +78,9 @@ def cpu_baseline(sample=(96, 48, 48), nlevels=4):
     """The oracle (numpy/scipy, FP64 GMG-PCG with CSR operators, Galerkin
     hierarchy built once per solve) on a bounded sample of the same workload:
     same boundary conditions and density generator at 96x48x48 (coarsest grid
-    12x6x6, as on the GPU at full size)."""
+    12x6x6, as on the GPU at full size).  Returns the timed solve (relres
+    1e-8) and, untimed, the same PCG continued to relres 1e-12 as the
+    oracle's reference solution for the device comparison."""
     from oracle import fem, mg
     from topoptlab_amd.workloads import gaussian_random_density
     nelx, nely, nelz = sample
@@ -92,7 +94,92 @@ def cpu_baseline(sample=(96, 48, 48), nlevels=4):
     h = mg.Hierarchy(K, nelx, nely, nelz, 3, fixed, nlevels)
     u, info, it = mg.pcg(K, f[free, 0], h.vcycle, rtol=1e-8)
     t = time.perf_counter() - t0
-    return t, it, ndof
+    u12, _, it12 = mg.pcg(K, f[free, 0], h.vcycle, x0=u, rtol=1e-12)
+    full = lambda v: np.bincount(free, weights=v, minlength=ndof)
+    return {"t": t, "its": it, "ndof": ndof, "x": x, "f": f[:, 0], "fixed": fixed,
+            "u8": full(u), "u12": full(u12), "its12": it + it12, "dims": sample,
+            "nlevels": nlevels}
+
+
+def device_vs_oracle(dev, base, steps=3):
+    """The CPU-baseline sample solved on the device with the same hierarchy
+    depth: ms per solve at that size, and the device's compliance and
+    displacement errors against the oracle's relres-1e-12 solution, for the
+    device FP64 V-cycle at relres 1e-8 (the benchmark setting) and 1e-12."""
+    import torch
+    from topoptlab_amd.elements import lk_linear_elast_3d
+    from topoptlab_amd.fem import StiffnessOperator
+    nelx, nely, nelz = base["dims"]
+    op = StiffnessOperator(nelx, nely, nelz, KE=lk_linear_elast_3d(), fixed=base["fixed"],
+                           device=dev)
+    x = torch.as_tensor(base["x"], device=dev)
+    f = torch.as_tensor(base["f"], device=dev)
+    c_ref = float(base["f"] @ base["u12"])
+    nref = np.linalg.norm(base["u12"])
+    out = {"grid": f"{nelx}x{nely}x{nelz}", "levels": base["nlevels"],
+           "oracle_reference": f"oracle FP64 GMG-PCG continued to relres 1e-12 "
+                               f"({base['its12']} iterations)"}
+
+    def solve(rtol):
+        op.set_density(x)
+        op.setup_mg(nlevels=base["nlevels"], precision="fp64")
+        return op.solve(f, rtol=rtol, preconditioner="gmg", check_every=16)
+
+    for rtol in (1e-8, 1e-12):
+        solve(rtol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            u, info = solve(rtol)
+        torch.cuda.synchronize()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        ud = u.cpu().numpy()
+        key = "rtol_1e-8" if rtol == 1e-8 else "rtol_1e-12"
+        out[key] = {"ms_per_solve": round(ms, 3), "pcg_iterations": info["iterations"],
+                    "relres_true": info["relres_true"],
+                    "compliance_rel_err_vs_oracle": abs(float(base["f"] @ ud) - c_ref) / c_ref,
+                    "u_rel_err_vs_oracle": float(np.linalg.norm(ud - base["u12"]) / nref)}
+        log(f"device at {nelx}x{nely}x{nelz}, relres {rtol:.0e}: {ms:.2f} ms, "
+            f"{info['iterations']} its, compliance err "
+            f"{out[key]['compliance_rel_err_vs_oracle']:.2e}, u err "
+            f"{out[key]['u_rel_err_vs_oracle']:.2e}")
+    c8 = float(base["f"] @ base["u8"])
+    out["oracle_rtol_1e-8"] = {"compliance_rel_err": abs(c8 - c_ref) / c_ref,
+                               "u_rel_err": float(np.linalg.norm(base["u8"] - base["u12"]) / nref)}
+    op.close()
+    return out
+
+
+def bench_c1(dev, iters=100):
+    """The 60x20 MBB OC loop (C1): 100 iterations without the early stop, on
+    the device (main, GMG-PCG to relres 1e-10) and in the oracle (COO
+    assembly, K[free][:, free], SuperLU spsolve, matrix filter, oc_top88) on
+    one host core; ms per iteration and the largest per-iteration objective
+    difference."""
+    import torch
+    from oracle import topopt
+    from topoptlab_amd.topology_optimization import main as tl_main
+    h = []
+    tl_main(60, 20, 0.5, 2.4, ft=1, nouteriter=5, stop_on_change=False, history=[], device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    tl_main(60, 20, 0.5, 2.4, ft=1, nouteriter=iters, stop_on_change=False, history=h,
+            device=dev)
+    torch.cuda.synchronize()
+    t_dev = time.perf_counter() - t0
+    h_ref = []
+    t0 = time.perf_counter()
+    topopt.topopt_loop(60, 20, 0.5, 2.4, ft=1, nouteriter=iters, stop_on_change=False,
+                       history=h_ref)
+    t_cpu = time.perf_counter() - t0
+    err = max(abs(a[0] - b[0]) / b[0] for a, b in zip(h, h_ref))
+    log(f"C1 60x20 MBB: device {1e3 * t_dev / iters:.2f} ms/iteration, oracle "
+        f"{1e3 * t_cpu / iters:.2f} ms/iteration, max objective rel err {err:.1e}")
+    return {"workload": "2D MBB 60x20 Q4, ft=1 rmin 2.4, OC, 100 iterations (no early stop)",
+            "ms_per_iteration": round(1e3 * t_dev / iters, 3),
+            "cpu_oracle_ms_per_iteration": round(1e3 * t_cpu / iters, 3),
+            "cpu_cores": 1, "objective_max_rel_err_vs_oracle": err,
+            "final_objective": h[-1][0]}
 
 
 def kernel_kinds(dims, vcycle, nlevels, fp64_mfma=False):
@@ -106,14 +193,20 @@ def kernel_kinds(dims, vcycle, nlevels, fp64_mfma=False):
     flops_ku = 1152.0 * nel
     vb = 8 if vcycle == "fp64" else 4  # V-cycle vector bytes (bf16/fp16: fp32 vectors)
     mfma = vcycle in ("bf16", "fp16")
+    # compute-bound kinds carry bound "mfma" (the compute roof) with the
+    # peak of the unit that runs them: FP64 vector FMAs and FP64 MFMA share
+    # 78.6 TFLOP/s on MI355X, FP32 vector FMAs 157.3
     if mfma:
         vk = f"kucol_kernel {vcycle.upper()}-input MFMA (fp32 accumulate)"
         vbound, vpeak = "hbm", HBM_PEAK_GBS
     elif vcycle == "fp64" and fp64_mfma:
         vk = "kucol64_kernel FP64 MFMA (v_mfma_f64_16x16x4f64)"
         vbound, vpeak = "mfma", FP64_PEAK_TFLOPS
+    elif vcycle == "fp64":
+        vk = "ku3d_kernel FP64 (vector FMAs)"
+        vbound, vpeak = "mfma", FP64_PEAK_TFLOPS
     else:
-        vk = f"ku3d_kernel {vcycle.upper()}"
+        vk = "ku3d_kernel FP32 (vector FMAs)"
         vbound, vpeak = "mfma", FP32_PEAK_TFLOPS
     sb = 14 * 9 * nn1 * vb  # level-1 block stencil bytes (forward half: centre + 13 blocks)
     return {
@@ -512,18 +605,30 @@ def main():
         variants["reference"] = {"vcycle": "fp64", "rtol": 1e-12,
                                  "pcg_iterations": inf_ref["iterations"], "compliance": c_ref}
 
-    cpu = None
+    cpu = same_size = c1 = None
     if not args.no_cpu_baseline:
-        tc, itc, ndof_s = cpu_baseline()
+        base = cpu_baseline()
+        tc, itc, ndof_s = base["t"], base["its"], base["ndof"]
         scale = ndof / ndof_s
+        log(f"cpu baseline sample: {tc:.2f} s for {ndof_s} DOF ({itc} its)")
+        same_size = device_vs_oracle(dev, base)
+        ratio = 1e3 * tc / same_size["rtol_1e-8"]["ms_per_solve"]
         cpu = {"value": round(1e3 * tc * scale, 1), "unit": "ms per FEM solve (extrapolated)",
                "cores": 1, "kind": "port",
                "sample": (f"oracle numpy/scipy FP64 GMG-PCG (4 levels, CSR operators, "
                           f"Galerkin hierarchy once per solve) on the same cantilever at 96x48x48 "
-                          f"({ndof_s} DOF): {tc:.2f} s incl. assembly, {itc} iterations; "
-                          f"scaled by the DOF ratio {scale:.1f} (linear-time assumption); "
-                          f"scipy sparse kernels run on 1 of {os.cpu_count()} host cores")}
-        log(f"cpu baseline sample: {tc:.2f} s for {ndof_s} DOF ({itc} its)")
+                          f"({ndof_s} DOF): {tc:.2f} s incl. assembly, {itc} iterations, measured "
+                          f"{ratio:.0f}x slower than the device's FP64 GMG-PCG on the same "
+                          f"96x48x48 problem; the value scales the sample by the DOF ratio "
+                          f"{scale:.1f} (linear-time assumption: the full-size solve needs "
+                          f"{iters} iterations on the device against {itc} at 96x48x48, so the "
+                          f"extrapolation favours the CPU); scipy sparse kernels run on 1 of "
+                          f"{os.cpu_count()} host cores"),
+               "same_size_ms": round(1e3 * tc, 1),
+               "same_size_speedup_of_device": round(ratio, 1)}
+        del base
+        if not args.no_variants:
+            c1 = bench_c1(dev)
 
     helmholtz = mma_step = None
     if not args.no_variants:
@@ -561,6 +666,8 @@ def main():
                    "compliance": compliance, "solve_hbm_TBps": round(solve_tbps, 3)},
         "roofline": roof,
         "variants": variants,
+        "vs_oracle_96x48x48": same_size,
+        "c1_mbb_60x20": c1,
         "batched": batched,
         "helmholtz_filter": helmholtz,
         "mma_update": mma_step,

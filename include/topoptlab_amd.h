@@ -172,11 +172,13 @@ int tl_filter_apply(tl_filter* f, const double* in, double* out, int32_t mode,
 int tl_helmholtz_transfer(int32_t nelx, int32_t nely, const double* in, double* out,
                           int32_t mode, void* stream);
 
-/* Optimality-criteria bisection of optimizer/optimality_criterion.py:9-79
- * (no element flags).  x, dc, dv, xnew: device n; g, gt: host scalars. */
+/* Optimality-criteria bisection of optimizer/optimality_criterion.py:9-79.
+ * x, dc, dv, xnew: device n; g, gt: host scalars.  el_flags (device n int32,
+ * may be NULL): 1 = passive element (xnew 0), 2 = active element (xnew 1),
+ * applied inside every bisection step as at optimality_criterion.py:68-71. */
 int tl_oc_update(int64_t n, const double* x, const double* dc,
-                 const double* dv, double g, double move, double l1, double l2,
-                 double* xnew, double* gt, void* stream);
+                 const double* dv, const int32_t* el_flags, double g, double move,
+                 double l1, double l2, double* xnew, double* gt, void* stream);
 
 /* Instrumentation (no counterpart in the reference): while enabled, every
  * launch of the fine-grid operator kernel and of the first coarse level's

@@ -232,7 +232,8 @@ def cantilever_2d_twoloads(nelx, nely, ndof):
 
 def cantilever_3d(nelx, nely, nelz, ndof, force_mode="linear"):
     """``example_bc/lin_elast.py:274-338``: wall at ix=0, z-symmetry on the
-    iz=0 layer, downward load on nodes (ix=nelx, iy=0) for all iz."""
+    iz=0 layer, downward load on the y-DOFs of nodes (ix=nelx, iy=nely) for
+    all iz (the last node of each node plane)."""
     f = np.zeros((ndof, 1))
     nxy = (nelx + 1) * (nely + 1)
     fix = np.arange(0, 3 * (nely + 1), 3)
@@ -340,8 +341,9 @@ def compliance_sens(KE, edof, u, xPhys, eps=1e-9, penal=3.0):
     return obj, dc
 
 
-def oc_top88(x, volfrac, dc, dv, g, move=0.2, l1=0.0, l2=1e9):
-    """``optimizer/optimality_criterion.py:9-79`` (no element flags)."""
+def oc_top88(x, volfrac, dc, dv, g, el_flags=None, move=0.2, l1=0.0, l2=1e9):
+    """``optimizer/optimality_criterion.py:9-79``; ``el_flags`` 1 holds an
+    element at 0 (passive), 2 at 1 (active), inside each step (68-71)."""
     xnew = np.zeros(x.shape)
     gt = g
     while (l2 - l1) / (l1 + l2) > 1e-3:
@@ -349,6 +351,9 @@ def oc_top88(x, volfrac, dc, dv, g, move=0.2, l1=0.0, l2=1e9):
         xnew[:] = np.maximum(0.0, np.maximum(
             x - move, np.minimum(1.0, np.minimum(x + move,
                                                  x * np.sqrt(-dc / dv / lmid)))))
+        if el_flags is not None:
+            xnew[el_flags == 1] = 0
+            xnew[el_flags == 2] = 1
         gt = g + np.sum(dv * (xnew - x))
         if gt > 0:
             l1 = lmid

@@ -16,14 +16,19 @@ from . import fem, mma
 
 def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
                 KE=None, eps=1e-9, penal=3.0, nouteriter=2000,
-                stop_on_change=True, history=None, filter_mode="matrix", optimizer="oc"):
+                stop_on_change=True, history=None, filter_mode="matrix", optimizer="oc",
+                el_flags=None):
     """Run the OC loop; return (x, obj) like ``topology_optimization.py:688``.
 
     ``history`` (a list) receives one (obj, vol, change) tuple per iteration,
     matching the log line of ``topology_optimization.py:669-670``.
     ``stop_on_change=False`` disables the ``change < 0.01`` break (672-673)
-    so that a fixed number of iterations runs.
+    so that a fixed number of iterations runs.  ``el_flags`` (OC only) is
+    passed to ``oc_top88`` as at 579-581; the initial design is not changed
+    for OC (the flag block at 248-259 belongs to the MMA branch).
     """
+    if el_flags is not None and optimizer != "oc":
+        raise NotImplementedError("element flags are restated for the OC branch only")
     ndim = 2 if nelz is None else 3
     n = int(np.prod([nelx, nely, nelz][:ndim]))
     if KE is None:
@@ -85,7 +90,7 @@ def topopt_loop(nelx, nely, volfrac, rmin, ft=1, nelz=None, bcs=fem.mbb_2d,
             x = out[0][:, 0].copy()
             xhist = (xhist + [x[:, None].copy()])[-4:]
         else:
-            x, g = fem.oc_top88(x, volfrac, dc, dv, g)
+            x, g = fem.oc_top88(x, volfrac, dc, dv, g, el_flags=el_flags)
         if ft == 1 and helm:
             xPhys = hfilt(x)
         elif ft == 1:

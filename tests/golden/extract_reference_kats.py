@@ -6,6 +6,7 @@ known-answer data of
   tests/test_compliance.py:8-113 end-to-end compliance goldens (filters,
                                  optimizers, heat)
   tests/test_linsolver.py:10-28  the 3x3 linear-system known answer
+  tests/test_passive.py:7-10     the passive-element (el_flags) golden
 and stores it as JSON.  Run once in a container that has the reference at
 /root/reference; the JSON is committed, so the tests never need the reference.
 """
@@ -60,6 +61,12 @@ def main():
     names, rows = _parametrize_rows(REF / "test_compliance.py", "test_compliance_heat")
     out["compliance_heat"] = {"names": names, "rows": rows,
                               "source": "tests/test_compliance.py:88-113 (3 decimals)"}
+    names, rows = _parametrize_rows(REF / "test_passive.py", "test_compliance_1")
+    out["compliance_passive"] = {
+        "names": names, "rows": rows,
+        "passive": "circle of element flags 1: center (nelx/3 - 1, nely/2 - 1), radius nely/3, "
+                   "element (i, j) = divmod(e, nely) inside when (i-cx)^2 + (j-cy)^2 <= r^2",
+        "source": "tests/test_passive.py:7-31 (assert to 2 decimals)"}
     out["linsystem_3x3"] = {"A": [[2, 1, 0], [1, 2, 0], [0, 0, 1]], "b": [1, 2, 0],
                             "x": [0.0, 1.0, 0.0], "source": "tests/test_linsolver.py:20-26"}
     OUT.write_text(json.dumps(out, indent=1))

@@ -40,4 +40,5 @@ def test_error_paths_without_gpu():
     assert b"ndim" in lib.tl_last_error()
     h = ctypes.c_void_p()
     assert lib.tl_filter_create(2, 4, 4, 0, -1.0, None, ctypes.byref(h)) == _lib.TL_EINVAL
-    assert lib.tl_oc_update(0, None, None, None, 0.0, 0.2, 0.0, 1e9, None, None, None) == _lib.TL_EINVAL
+    assert lib.tl_oc_update(0, None, None, None, None, 0.0, 0.2, 0.0, 1e9, None, None,
+                            None) == _lib.TL_EINVAL

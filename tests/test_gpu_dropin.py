@@ -1,0 +1,257 @@
+"""GPU tests of the drop-in surface: ``solve_lin`` / ``pcg`` called the way
+topoptlab calls them, passive/active element flags, the SimpleLogger file
+format of ``main``, and the device GMG-PCG on the benchmark's boundary
+conditions against the oracle's direct solve (marked ``gpu``)."""
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+from scipy.sparse.linalg import spsolve
+
+from oracle import fem, topopt
+from tests.conftest import GOLDEN, rel
+from topoptlab_amd import example_bc
+from topoptlab_amd.workloads import gaussian_random_density
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[case]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def _twoload_problem(gpu, nelx=64, nely=32, seed=3):
+    """2D cantilever with two load cases (example_bc/lin_elast.py:141-182) on
+    a seeded random density: the device operator and the oracle's K_ff."""
+    from topoptlab_amd.elements import lk_linear_elast_2d
+    from topoptlab_amd.fem import StiffnessOperator
+    ndof = 2 * (nelx + 1) * (nely + 1)
+    f, fixed, free = fem.cantilever_2d_twoloads(nelx, nely, ndof)
+    x = 0.001 + 0.999 * np.random.default_rng(seed).random(nelx * nely)
+    K = fem.apply_bc(fem.assemble_K(fem.ke_elast_2d(), fem.edofmat_2d(nelx, nely, 2),
+                                    fem.simp(x), ndof), free).tocsc()
+    op = StiffnessOperator(nelx, nely, None, KE=lk_linear_elast_2d(), fixed=fixed, device=gpu)
+    op.set_density(x)
+    return op, K, f[free], free
+
+
+def test_solve_lin_reference_call_pattern(gpu):
+    """solve_linsystem.py:341-352 as main calls it (topology_optimization.py:
+    451-487): rhs (nfree, nbc), rhs0 = u[free] warm start, P handed back and
+    reused by the next solve of the same density."""
+    from topoptlab_amd.solve_linsystem import solve_lin
+    say("solve_lin reference call pattern")
+    op, K, rhs, free = _twoload_problem(gpu)
+    ref = spsolve(K, rhs)
+    rhs0 = np.zeros_like(rhs)
+    sol, fact, P = solve_lin(op, rhs, solver="amd-gmgpcg", rhs0=rhs0,
+                             solver_kw={"rtol": 1e-12}, preconditioner_kw={"precision": "fp64"})
+    assert fact is None and P is op
+    assert sol.shape == rhs.shape
+    its_cold = [i["iterations"] for i in op.last_infos]
+    for k in range(rhs.shape[1]):
+        assert rel(sol[:, k].cpu().numpy(), ref[:, k]) < 1e-8
+    # warm start from the solution, hierarchy reused through P
+    key = op._mg_key
+    sol2, _, P2 = solve_lin(op, rhs, solver="amd-gmgpcg", rhs0=sol, solver_kw={"rtol": 1e-12},
+                            P=P)
+    assert P2 is op and op._mg_key == key
+    its_warm = [i["iterations"] for i in op.last_infos]
+    assert all(w < c for w, c in zip(its_warm, its_cold)), (its_warm, its_cold)
+    for k in range(rhs.shape[1]):
+        assert rel(sol2[:, k].cpu().numpy(), ref[:, k]) < 1e-8
+    # the Jacobi-preconditioned device PCG reaches the same solution
+    sol3, _, _ = solve_lin(op, rhs, solver="amd-pcg", solver_kw={"rtol": 1e-12})
+    for k in range(rhs.shape[1]):
+        assert rel(sol3[:, k].cpu().numpy(), ref[:, k]) < 1e-7
+    op.close()
+
+
+def test_solve_lin_errors_and_cycle_options(gpu):
+    """Unknown solver -> ValueError (solve_linsystem.py:364-365); maxiter hit
+    -> RuntimeError (349-351); gamma/gamma_from reach the hierarchy (a W-cycle
+    needs fewer iterations than the V-cycle); unknown options raise."""
+    from topoptlab_amd.solve_linsystem import solve_lin
+    say("solve_lin errors and cycle options")
+    op, K, rhs, free = _twoload_problem(gpu, seed=4)
+    with pytest.raises(ValueError, match="Unknown solver"):
+        solve_lin(op, rhs, solver="no-such-solver")
+    with pytest.raises(RuntimeError, match="did not converge"):
+        solve_lin(op, rhs, solver="amd-gmgpcg", solver_kw={"rtol": 1e-12, "maxiter": 2})
+    with pytest.raises(ValueError, match="preconditioner_kw"):
+        solve_lin(op, rhs, solver="amd-gmgpcg", preconditioner_kw={"smoother": "sor"})
+    with pytest.raises(ValueError, match="solver_kw"):
+        solve_lin(op, rhs, solver="amd-gmgpcg", solver_kw={"tol": 1e-3})
+    op.set_density(np.full(op.nel, 0.5))
+    solve_lin(op, rhs, solver="amd-gmgpcg", solver_kw={"rtol": 1e-10},
+              preconditioner_kw={"nlevels": 4, "precision": "fp64"})
+    its_v = [i["iterations"] for i in op.last_infos]
+    assert op._mg_key[-2:] == (1, 1)
+    op.set_density(np.full(op.nel, 0.5))
+    solve_lin(op, rhs, solver="amd-gmgpcg", solver_kw={"rtol": 1e-10},
+              preconditioner_kw={"nlevels": 4, "precision": "fp64", "gamma": 2, "gamma_from": 1})
+    its_w = [i["iterations"] for i in op.last_infos]
+    assert op._mg_key[-2:] == (2, 1)
+    assert all(w < v for w, v in zip(its_w, its_v)), (its_w, its_v)
+    op.close()
+
+
+def test_pcg_reference_argument_order(gpu):
+    """linear_solvers.py:50-59: pcg(A, b, P, x0, rtol, atol, maxiter) with
+    free-DOF vectors; info 0 on convergence, maxiter-1 when it runs out."""
+    from topoptlab_amd.linear_solvers import pcg
+    from topoptlab_amd.log_utils import SimpleLogger
+    say("pcg reference argument order")
+    op, K, rhs, free = _twoload_problem(gpu, seed=5)
+    b = rhs[:, 0]
+    ref = spsolve(K, b)
+    x, info = pcg(op, b, "gmg", np.zeros_like(b), 1e-12, 0.0, 500)
+    assert info == 0
+    assert x.shape == (len(free),)
+    assert rel(x.cpu().numpy(), ref) < 1e-8
+    x, info = pcg(op, b, "jacobi", None, 1e-12, 0.0, 3)
+    assert info == 2
+    x, info = pcg(op, b, op, None, 1e-12, 0.0, 500)  # P = the operator: its GMG hierarchy
+    assert info == 0 and rel(x.cpu().numpy(), ref) < 1e-8
+    op.close()
+
+
+def test_c2_boundary_conditions_against_direct_solve_small(gpu):
+    """The benchmark's cantilever (x=0 face clamped, unit line load on the
+    bottom edge of x=L) on a seeded random field at 32x16x16: device FP64
+    GMG-PCG to relres 1e-12 against the oracle's spsolve."""
+    from topoptlab_amd.elements import lk_linear_elast_3d
+    from topoptlab_amd.fem import StiffnessOperator
+    import torch
+    say("C2 boundary conditions 32x16x16 vs spsolve")
+    nelx, nely, nelz = 32, 16, 16
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    x = gaussian_random_density(nelx, nely, nelz, seed=0)
+    f, fixed, free = fem.clamped_lineload_3d(nelx, nely, nelz, ndof)
+    K = fem.apply_bc(fem.assemble_K(fem.ke_elast_3d(), fem.edofmat_3d(nelx, nely, nelz, 3),
+                                    fem.simp(x), ndof), free).tocsc()
+    uref = spsolve(K, f[free, 0], permc_spec="MMD_AT_PLUS_A")
+    op = StiffnessOperator(nelx, nely, nelz, KE=lk_linear_elast_3d(), fixed=fixed, device=gpu)
+    op.set_density(x)
+    op.setup_mg(nlevels=3, precision="fp64")
+    u, info = op.solve(torch.as_tensor(f[:, 0], device=gpu), rtol=1e-12)
+    ud = u.cpu().numpy()
+    c_ref = f[free, 0] @ uref
+    assert abs(f[:, 0] @ ud - c_ref) / c_ref < 1e-8
+    assert rel(ud[free], uref) < 1e-7
+    op.close()
+
+
+def test_c2_boundary_conditions_against_direct_solve_fixture(gpu):
+    """Same boundary conditions at 48x24x24 against the committed direct
+    solution (tests/golden/make_c2bc_fixture.py): compliance within 1e-8,
+    sampled displacements within 1e-9 (1e-6 at relres 1e-8), for the plain FP64 V-cycle at relres
+    1e-12 and 1e-8 and the FP32 W-cycle at relres 1e-8."""
+    import hashlib
+    import torch
+    from topoptlab_amd.elements import lk_linear_elast_3d
+    from topoptlab_amd.fem import StiffnessOperator
+    say("C2 boundary conditions 48x24x24 vs direct-solve fixture")
+    fx = np.load(GOLDEN / "c2bc_48x24x24_seed0.npz")
+    nelx, nely, nelz = (int(v) for v in fx["dims"])
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    x = gaussian_random_density(nelx, nely, nelz, seed=0)
+    assert hashlib.sha256(x.tobytes()).hexdigest() == str(fx["density_sha256"])
+    _, f, fixed, free, _ = example_bc.clamped_lineload_3d(nelx, nely, nelz, ndof)
+    op = StiffnessOperator(nelx, nely, nelz, KE=lk_linear_elast_3d(), fixed=fixed, device=gpu)
+    ft = torch.as_tensor(f[:, 0], device=gpu)
+    idx = fx["idx"]
+    scale = np.abs(fx["u_idx"]).max()
+    for vc, gamma, rtol, ctol, utol in [("fp64", 1, 1e-12, 1e-10, 1e-9),
+                                        ("fp64", 1, 1e-8, 1e-8, 1e-6),
+                                        ("fp32", 2, 1e-8, 1e-8, 1e-6)]:
+        op.set_density(x)
+        op.setup_mg(nlevels=3, precision=vc, gamma=gamma, gamma_from=1)
+        u, info = op.solve(ft, rtol=rtol)
+        ud = u.cpu().numpy()
+        c_err = abs(f[:, 0] @ ud - float(fx["compliance"])) / float(fx["compliance"])
+        u_err = np.abs(ud[idx] - fx["u_idx"]).max() / scale
+        print(f"{vc} gamma {gamma} rtol {rtol}: {info['iterations']} its, compliance err "
+              f"{c_err:.2e}, sampled u err {u_err:.2e}", flush=True)
+        assert info["relres_true"] < 2 * rtol
+        assert c_err < ctol
+        assert u_err < utol
+    op.close()
+
+
+def _circle_flags(nelx, nely):
+    """The passive region of tests/test_passive.py:16-19 (geometries.sphere):
+    element e at (i, j) = divmod(e, nely) is flagged 1 when
+    (i - (nelx/3 - 1))^2 + (j - (nely/2 - 1))^2 <= (nely/3)^2."""
+    i, j = np.divmod(np.arange(nelx * nely), nely)
+    cx, cy, r = nelx / 3 - 1, nely / 2 - 1, nely / 3
+    return np.where((i - cx) ** 2 + (j - cy) ** 2 <= r ** 2, 1, 0).astype(np.int32)
+
+
+def test_oc_update_with_flags_bit_exact(gpu):
+    import torch
+    from topoptlab_amd.optimizer import oc_top88
+    say("oc update with element flags")
+    rng = np.random.default_rng(11)
+    n = 5000
+    x = rng.random(n)
+    dc = -rng.random(n) * 10 ** rng.uniform(-3, 1, n)
+    dv = 0.5 + rng.random(n)
+    flags = rng.integers(0, 3, n).astype(np.int32)
+    xr, gr = fem.oc_top88(x, 0.5, dc, dv, 0.1, el_flags=flags)
+    xd, gd = oc_top88(torch.as_tensor(x, device=gpu), 0.5, torch.as_tensor(dc, device=gpu),
+                      torch.as_tensor(dv, device=gpu), 0.1, el_flags=flags)
+    xd = xd.cpu().numpy()
+    assert np.array_equal(xd, xr)
+    assert (xd[flags == 1] == 0).all() and (xd[flags == 2] == 1).all()
+    assert gd == pytest.approx(gr, rel=1e-12, abs=1e-12)
+
+
+def test_main_passive_golden(gpu, kats):
+    """tests/test_passive.py:7-31: 60x20 cantilever_2d, ft=0, a passive
+    circle; golden 221.9720563724 (the reference asserts 2 decimals)."""
+    from topoptlab_amd.topology_optimization import main
+    say("main passive golden")
+    row = kats["compliance_passive"]["rows"][0]
+    nelx, nely, volfrac, ft, rmin = row[:5]
+    flags = _circle_flags(nelx, nely)
+    h_ref = []
+    _, obj_ref = topopt.topopt_loop(nelx, nely, volfrac, rmin, ft=ft, bcs=fem.cantilever_2d,
+                                    el_flags=flags, history=h_ref)
+    h = []
+    x, obj = main(nelx, nely, volfrac, rmin, ft=ft, bcs=example_bc.cantilever_2d,
+                  el_flags=flags, lin_solver_kw={"name": "amd-gmgpcg", "rtol": 1e-12},
+                  history=h, device=gpu)
+    assert obj == pytest.approx(row[-1], abs=5e-3)
+    assert len(h) == len(h_ref)
+    assert abs(obj - obj_ref) / obj_ref < 1e-8
+    assert (x[flags == 1] == 0).all()
+
+
+def test_main_log_file_format(gpu, tmp_path):
+    """main(output_kw={"write_log": True}) writes the header of
+    topology_optimization.py:180-199 and the iteration lines of 669-670;
+    parse_simple_logfile (log_utils.py:449-560) reads them back."""
+    from topoptlab_amd.log_utils import parse_simple_logfile
+    from topoptlab_amd.topology_optimization import main
+    say("main log file format")
+    base = str(tmp_path / "mbb")
+    h = []
+    main(60, 20, 0.5, 2.4, ft=1, nouteriter=12, history=h, device=gpu,
+         output_kw={"file": base, "write_log": True, "export": False, "display": False})
+    text = Path(base + ".log").read_text().splitlines()
+    assert text[:6] == ["optimizer oc", "number of spatial dimensions: 2", "elements: 60 x 20",
+                        "volfrac: 0.5 rmin: 2.4", "filter: Density based",
+                        "filter mode: matrix"]
+    assert text[6].startswith("it.: 1 obj.: ")
+    log = parse_simple_logfile(base + ".log")
+    hd = log["header"]
+    assert (hd["nelx"], hd["nely"]) == (60, 20)
+    assert hd["volfrac"] == 0.5 and hd["rmin"] == 2.4 and hd["mode"] == "matrix"
+    hist = log["history"]
+    assert np.array_equal(hist["it"], np.arange(1, 13))
+    assert np.allclose(hist["obj"], [o for o, _, _ in h], rtol=0, atol=1e-10 * 1e3)
+    assert np.allclose(hist["vol"], [v for _, v, _ in h], rtol=0, atol=1e-10)
+    assert np.allclose(hist["ch"], [c for _, _, c in h], rtol=0, atol=1e-10)

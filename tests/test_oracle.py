@@ -124,6 +124,33 @@ def test_bcs_match_package():
     assert list(fixed) == [18, 19, 20, 21, 22]
 
 
+@pytest.mark.parametrize("nelx,nely,nelz", [(4, 3, 2), (5, 2, 6)])
+@pytest.mark.parametrize("force_mode", ["linear", "constant"])
+def test_cantilever_3d_closed_form(nelx, nely, nelz, force_mode):
+    """example_bc/lin_elast.py:274-338 stated node by node: all 3 DOFs of
+    the nodes with ix = 0 fixed (the wall), plus the z-DOF of every node of
+    the iz = 0 layer (z-symmetry); the only loads are on the y-DOFs of the
+    nodes (ix = nelx, iy = nely, iz), -(1 - iz/nelz) ("linear") or -1
+    ("constant").  Node id (nelx+1)(nely+1) iz + (nely+1) ix + iy."""
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    fixed_cf, f_cf = set(), np.zeros(ndof)
+    for iz in range(nelz + 1):
+        for ix in range(nelx + 1):
+            for iy in range(nely + 1):
+                node = (nelx + 1) * (nely + 1) * iz + (nely + 1) * ix + iy
+                if ix == 0:
+                    fixed_cf |= {3 * node, 3 * node + 1, 3 * node + 2}
+                if iz == 0:
+                    fixed_cf.add(3 * node + 2)
+                if ix == nelx and iy == nely:
+                    f_cf[3 * node + 1] = -1.0 if force_mode == "constant" else -(1 - iz / nelz)
+    f, fixed, free = fem.cantilever_3d(nelx, nely, nelz, ndof, force_mode=force_mode)
+    assert list(fixed) == sorted(fixed_cf)
+    assert list(free) == sorted(set(range(ndof)) - fixed_cf)
+    assert np.allclose(f[:, 0], f_cf, rtol=0, atol=1e-15)
+    assert np.count_nonzero(f) == (nelz + 1 if force_mode == "constant" else nelz)
+
+
 def test_c2_bc_counts():
     nelx, nely, nelz = 192, 96, 96
     ndof = 3 * 193 * 97 * 97
@@ -343,3 +370,59 @@ def test_mma_subproblem_matches_slsqp():
         assert res.success
         assert np.abs(xm - res.x).max() < 1e-5, it
         assert np.all(xm >= alfa - 1e-12) and np.all(xm <= beta + 1e-12)
+
+
+def test_mma_adaptive_asymptotes_hand_worked():
+    """Svanberg's asymptote update for iter > 2 (IJNME 24 (1987), sec. 3,
+    and the defaults of optimizer/mma_utils.py:8-60: asyincr 1.2, asydecr
+    0.7, asymin 0.01, asymax 10, xmin 0, xmax 1), worked out by hand per
+    element: the factor is 1.2 when (x - xold1)(xold1 - xold2) > 0, 0.7 when
+    < 0, 1 when 0; low = x - factor (xold1 - low_prev), upp = x + factor
+    (upp_prev - xold1); then low is clamped to [x - 10, x - 0.01] and upp to
+    [x + 0.01, x + 10].  For iter <= 2 the asymptotes restart at x -/+ 0.5."""
+    from oracle import mma
+    x = np.full((5, 1), 0.5)
+    xold1 = np.array([[0.4], [0.6], [0.5], [0.499], [0.6]])
+    xold2 = np.array([[0.3], [0.4], [0.3], [0.498], [0.7]])
+    low_prev = np.array([[0.2], [0.3], [0.1], [0.495], [-20.0]])
+    upp_prev = np.array([[0.9], [0.8], [0.95], [0.5], [30.0]])
+    # 0: growing steps -> 0.5 - 1.2*0.2 = 0.26, 0.5 + 1.2*0.5 = 1.1
+    # 1: oscillation  -> 0.5 - 0.7*0.3 = 0.29, 0.5 + 0.7*0.2 = 0.64
+    # 2: no step      -> 0.5 - 0.4 = 0.1, 0.5 + 0.45 = 0.95
+    # 3: asymin       -> 0.4952 -> 0.49, 0.5012 -> 0.51
+    # 4: asymax       -> -24.22 -> -9.5, 35.78 -> 10.5
+    low_hand = [0.26, 0.29, 0.1, 0.49, -9.5]
+    upp_hand = [1.1, 0.64, 0.95, 0.51, 10.5]
+    kw = mma.mma_defaultkws(5, 1)
+    args = dict(move=kw["move"], asyinit=kw["asyinit"], asydecr=kw["asydecr"],
+                asyincr=kw["asyincr"], asymin=kw["asymin"], asymax=kw["asymax"])
+    df0 = -np.linspace(1.0, 2.0, 5)[:, None]
+    dfdx = np.full((1, 5), 0.2)
+    for it, lo_ref, up_ref in [(3, low_hand, upp_hand), (2, [0.0] * 5, [1.0] * 5)]:
+        out = mma.mmasub(1, 5, it, x, kw["xmin"], kw["xmax"], xold1, xold2, 0.0, df0,
+                         np.array([[0.0]]), dfdx, low_prev, upp_prev, kw["a0"], kw["a"],
+                         kw["c"], kw["d"], **args)
+        assert np.allclose(out[9][:, 0], lo_ref, rtol=0, atol=1e-15), it
+        assert np.allclose(out[10][:, 0], up_ref, rtol=0, atol=1e-15), it
+
+
+def test_passive_golden(kats):
+    """tests/test_passive.py:7-31: 60x20 cantilever_2d, ft=0, rmin 2.4, a
+    passive circle (flags 1 inside (i - (nelx/3 - 1))^2 + (j - (nely/2 -
+    1))^2 <= (nely/3)^2, (i, j) = divmod(e, nely)); golden 221.9720563724
+    (2 decimals in the reference)."""
+    row = kats["compliance_passive"]["rows"][0]
+    nelx, nely, volfrac, ft, rmin, mode, bcs, ref = row
+    assert (mode, bcs) == ("matrix", "cantilever_2d")
+    i, j = np.divmod(np.arange(nelx * nely), nely)
+    inside = (i - (nelx / 3 - 1)) ** 2 + (j - (nely / 2 - 1)) ** 2 <= (nely / 3) ** 2
+    flags = np.where(inside, 1, 0).astype(np.int32)
+    assert flags.sum() == 137
+    x, obj = topopt.topopt_loop(nelx, nely, volfrac, rmin, ft=ft, bcs=fem.cantilever_2d,
+                                el_flags=flags)
+    assert obj == pytest.approx(ref, abs=5e-3)
+    assert abs(obj - ref) < 1e-8
+    assert (x[flags == 1] == 0).all()
+    # without the flags the loop converges elsewhere: the flags matter
+    _, obj_free = topopt.topopt_loop(nelx, nely, volfrac, rmin, ft=ft, bcs=fem.cantilever_2d)
+    assert abs(obj_free - ref) > 1.0

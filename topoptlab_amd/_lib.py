@@ -81,7 +81,8 @@ _SIGS = {
     "tl_filter_apply": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "tl_helmholtz_transfer": (C.c_int, [C.c_int32, C.c_int32, C.c_void_p, C.c_void_p, C.c_int32,
                                         C.c_void_p]),
-    "tl_oc_update": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double,
+    "tl_oc_update": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                               C.c_double,
                                C.c_double, C.c_double, C.c_double, C.c_void_p,
                                C.POINTER(C.c_double), C.c_void_p]),
     "tl_profile": (C.c_int, [C.c_void_p, C.c_int32]),

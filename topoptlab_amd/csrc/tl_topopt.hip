@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 namespace tl {
@@ -182,7 +183,8 @@ struct OcState {
 
 __global__ __launch_bounds__(BS) void oc_step_kernel(const double* __restrict__ x,
                                                      const double* __restrict__ dc,
-                                                     const double* __restrict__ dv, int64_t n,
+                                                     const double* __restrict__ dv,
+                                                     const int32_t* __restrict__ flags, int64_t n,
                                                      double* __restrict__ xnew, const OcState* st,
                                                      double* partials) {
   if (st->done) return;
@@ -194,7 +196,11 @@ __global__ __launch_bounds__(BS) void oc_step_kernel(const double* __restrict__ 
   for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
     const double xi = x[i];
     const double cand = xi * sqrt(-dc[i] / dv[i] / lmid);
-    const double xn = fmax(0.0, fmax(xi - move, fmin(1.0, fmin(xi + move, cand))));
+    double xn = fmax(0.0, fmax(xi - move, fmin(1.0, fmin(xi + move, cand))));
+    if (flags) {  // passive (1) elements are 0, active (2) elements are 1 (:68-71)
+      const int32_t fl = flags[i];
+      xn = fl == 1 ? 0.0 : (fl == 2 ? 1.0 : xn);
+    }
     xnew[i] = xn;
     // explicit rounding ops: a contracted FMA would break the two-sum
     const double term = __dmul_rn(dv[i], __dsub_rn(xn, xi));
@@ -270,8 +276,12 @@ __global__ __launch_bounds__(BS) void mean_maxdiff_fold_kernel(const double* __r
 
 unsigned nblk(int64_t n) { return unsigned(std::max<int64_t>(1, (n + BS - 1) / BS)); }
 
-struct Scratch {  // process-wide reduction scratch for the handle-free calls
+// Process-wide reduction scratch of the handle-free calls, one per device.
+// The mutex is held for the whole call (allocation and use), so host threads
+// that call tl_oc_update / tl_mean_maxdiff concurrently are serialised.
+struct Scratch {
   DevBuf partials, st, out;
+  std::mutex mu;
   int ensure() {
     if (!partials.ptr) {
       TL_TRY(partials.alloc(sizeof(double) * 2 * kMaxRedBlocks));
@@ -287,6 +297,35 @@ Scratch& scratch_for_device() {
   int dev = 0;
   (void)hipGetDevice(&dev);
   return s[dev & 63];
+}
+
+// The bisection of optimality_criterion.py:61-77 on `stream`, with the
+// block partials (two values, kMaxRedBlocks stride, at most kOcBlocks
+// blocks) and the state in caller-provided device memory.
+constexpr unsigned kOcBlocks = 1024;
+
+int oc_bisect(int64_t n, const double* x, const double* dc, const double* dv,
+              const int32_t* el_flags, double g, double move, double l1, double l2,
+              double* xnew, double* gt, hipStream_t s, double* partials, OcState* st) {
+  OcState h{l1, l2, g, g, move, 0, 0};
+  h.done = ((l2 - l1) / (l1 + l2) > 1e-3) ? 0 : 1;
+  TL_HIP(hipMemcpyAsync(st, &h, sizeof(OcState), hipMemcpyHostToDevice, s));
+  const unsigned nb = unsigned(std::min<int64_t>(nblk(n), kOcBlocks));
+  for (int launched = 0; launched < 400;) {
+    for (int k = 0; k < 16; ++k) {
+      hipLaunchKernelGGL(oc_step_kernel, dim3(nb), dim3(BS), 0, s, x, dc, dv, el_flags, n, xnew,
+                         st, partials);
+      hipLaunchKernelGGL(oc_fold_kernel, dim3(1), dim3(BS), 0, s, partials, nb, st);
+    }
+    TL_CHECK_LAUNCH();
+    launched += 16;
+    TL_HIP(hipMemcpyAsync(&h, st, sizeof(OcState), hipMemcpyDeviceToHost, s));
+    TL_HIP(hipStreamSynchronize(s));
+    if (h.done) break;
+  }
+  TL_ARG(h.done, "OC bisection did not terminate within 400 steps");
+  *gt = h.gt;
+  return TL_OK;
 }
 
 }  // namespace
@@ -445,39 +484,23 @@ int tl_compliance_sens(tl_problem* P, const double* u, int32_t ncases, const dou
   return TL_OK;
 }
 
-int tl_oc_update(int64_t n, const double* x, const double* dc, const double* dv, double g,
-                 double move, double l1, double l2, double* xnew, double* gt, void* stream) {
+int tl_oc_update(int64_t n, const double* x, const double* dc, const double* dv,
+                 const int32_t* el_flags, double g, double move, double l1, double l2,
+                 double* xnew, double* gt, void* stream) {
   TL_ARG(x && dc && dv && xnew && gt, "null argument");
   TL_ARG(n > 0, "n must be positive");
   Scratch& S = scratch_for_device();
+  std::lock_guard<std::mutex> lock(S.mu);
   TL_TRY(S.ensure());
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  OcState h{l1, l2, g, g, move, 0, 0};
-  h.done = ((l2 - l1) / (l1 + l2) > 1e-3) ? 0 : 1;
-  TL_HIP(hipMemcpyAsync(S.st.ptr, &h, sizeof(OcState), hipMemcpyHostToDevice, s));
-  const unsigned nb = unsigned(std::min<int64_t>(nblk(n), 1024));
-  for (int launched = 0; launched < 400;) {
-    for (int k = 0; k < 16; ++k) {
-      hipLaunchKernelGGL(oc_step_kernel, dim3(nb), dim3(BS), 0, s, x, dc, dv, n, xnew,
-                         S.st.as<OcState>(), S.partials.as<double>());
-      hipLaunchKernelGGL(oc_fold_kernel, dim3(1), dim3(BS), 0, s, S.partials.as<double>(), nb,
-                         S.st.as<OcState>());
-    }
-    TL_CHECK_LAUNCH();
-    launched += 16;
-    TL_HIP(hipMemcpyAsync(&h, S.st.ptr, sizeof(OcState), hipMemcpyDeviceToHost, s));
-    TL_HIP(hipStreamSynchronize(s));
-    if (h.done) break;
-  }
-  TL_ARG(h.done, "OC bisection did not terminate within 400 steps");
-  *gt = h.gt;
-  return TL_OK;
+  return oc_bisect(n, x, dc, dv, el_flags, g, move, l1, l2, xnew, gt,
+                   static_cast<hipStream_t>(stream), S.partials.as<double>(), S.st.as<OcState>());
 }
 
 int tl_mean_maxdiff(int64_t n, const double* a, const double* b, double* mean, double* maxdiff,
                     void* stream) {
   TL_ARG(a && mean && n > 0, "null argument");
   Scratch& S = scratch_for_device();
+  std::lock_guard<std::mutex> lock(S.mu);
   TL_TRY(S.ensure());
   hipStream_t s = static_cast<hipStream_t>(stream);
   const unsigned nb = unsigned(std::min<int64_t>(nblk(n), 1024));

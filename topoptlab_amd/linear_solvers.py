@@ -6,19 +6,24 @@ device :class:`~topoptlab_amd.fem.StiffnessOperator` and ``P`` names the
 preconditioner ("gmg", "jacobi", None) or is the operator itself (GMG set up
 on it).  The stop rule is the reference's: recursive residual
 ``||r||_2 < max(atol, rtol*||b||_2)`` checked before each iteration.
-Returns ``(x, info)`` with info 0 on convergence, ``maxiter-1`` otherwise.
+Returns ``(x, info)`` with info 0 on convergence, ``maxiter-1`` otherwise
+(the loop index the reference returns when the loop runs out).  ``logger``
+receives ``PCGit. <iterations>`` on convergence, as at 118-119.
 """
 from __future__ import annotations
 
 import torch
 
 from .fem import StiffnessOperator
+from .log_utils import EmptyLogger
 
 
 def pcg(A: StiffnessOperator, b, P="gmg", x0=None, rtol=1e-5, atol=0.0, maxiter=100000,
-        check_every=8, **kwargs):
+        conv_criterium=None, conv_args=None, logger=EmptyLogger(), check_every=8, **kwargs):
     if not isinstance(A, StiffnessOperator):
         raise TypeError("Matrix A must be a topoptlab_amd StiffnessOperator.")
+    if conv_criterium is not None:
+        raise NotImplementedError("the device PCG stops on ||r||_2 < max(atol, rtol*||b||_2)")
     prec = "gmg" if P is A else P
     b = torch.as_tensor(b, dtype=torch.float64, device=A.device).reshape(-1)
     full = b.numel() == A.ndof
@@ -30,4 +35,7 @@ def pcg(A: StiffnessOperator, b, P="gmg", x0=None, rtol=1e-5, atol=0.0, maxiter=
     u, info = A.solve(f, u0=u0, rtol=rtol, atol=atol, maxiter=maxiter, preconditioner=prec,
                       check_every=check_every, raise_on_fail=False)
     x = u if full else u[A._free_t]
-    return x, (0 if info["status"] == 0 else maxiter - 1)
+    if info["status"] == 0:
+        logger.perf(f"PCGit. {info['iterations']}")
+        return x, 0
+    return x, maxiter - 1

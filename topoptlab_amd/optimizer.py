@@ -17,21 +17,28 @@ from ._lib import check, lib
 
 
 def oc_top88(x, volfrac, dc, dv, g, el_flags=None, move=0.2, l1=0.0, l2=1e9):
-    if el_flags is not None:
-        raise NotImplementedError("element flags (passive/active elements) are not supported")
+    """``el_flags`` (nel integers or None): 1 marks a passive element (held
+    at 0), 2 an active one (held at 1), set inside every bisection step as at
+    optimality_criterion.py:68-71."""
     x = torch.as_tensor(x, dtype=torch.float64).reshape(-1).contiguous()
     dev = x.device
     dc = torch.as_tensor(dc, dtype=torch.float64, device=dev).reshape(-1).contiguous()
     dv = torch.as_tensor(dv, dtype=torch.float64, device=dev).reshape(-1).contiguous()
     if not (x.numel() == dc.numel() == dv.numel()):
         raise ValueError("x, dc and dv must have the same length")
+    flags = None
+    if el_flags is not None:
+        flags = torch.as_tensor(el_flags, device=dev).reshape(-1).to(torch.int32).contiguous()
+        if flags.numel() != x.numel():
+            raise ValueError("el_flags must have one entry per element")
     xnew = torch.empty_like(x)
     gt = C.c_double()
     stream = torch.cuda.current_stream(dev).cuda_stream
     check(lib().tl_oc_update(x.numel(), C.c_void_p(x.data_ptr()), C.c_void_p(dc.data_ptr()),
-                             C.c_void_p(dv.data_ptr()), float(g), float(move), float(l1),
-                             float(l2), C.c_void_p(xnew.data_ptr()), C.byref(gt),
-                             C.c_void_p(stream)), "tl_oc_update")
+                             C.c_void_p(dv.data_ptr()),
+                             C.c_void_p(0 if flags is None else flags.data_ptr()), float(g),
+                             float(move), float(l1), float(l2), C.c_void_p(xnew.data_ptr()),
+                             C.byref(gt), C.c_void_p(stream)), "tl_oc_update")
     return xnew, gt.value
 
 

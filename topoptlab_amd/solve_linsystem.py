@@ -12,6 +12,14 @@ Solver names follow the reference's "library-solvername" scheme:
 density; ``rhs``/``rhs0`` are (nfree, nbc) arrays or tensors.  Unknown solver
 names raise ``ValueError("Unknown solver: ", solver)`` and non-convergence
 raises ``RuntimeError`` as in the reference (349-351).
+
+``preconditioner_kw`` of ``"amd-gmgpcg"`` takes the hierarchy options of
+``StiffnessOperator.setup_mg`` (nlevels, omega, nu1, nu2, precision,
+max_coarse_dof, gamma, gamma_from); other keys raise ``ValueError``.  The
+returned ``P`` is the operator with its hierarchy; passing it back (the
+adjoint solve of topology_optimization.py:479-487) reuses the hierarchy
+while the density is unchanged.  ``solver_kw`` takes rtol, atol, maxiter and
+check_every.
 """
 from __future__ import annotations
 
@@ -20,20 +28,25 @@ from typing import Any, Dict, Optional
 import torch
 
 from .fem import StiffnessOperator
+from .log_utils import EmptyLogger
 
 SOLVERS = ("amd-pcg", "amd-gmgpcg")
+MG_KEYS = ("nlevels", "omega", "nu1", "nu2", "precision", "max_coarse_dof", "gamma",
+           "gamma_from")
 
 
 def _solver_opts(solver_kw: Dict) -> Dict:
     kw = dict(solver_kw)
     out = {"rtol": kw.pop("rtol", 1e-5), "atol": kw.pop("atol", 0.0),
            "maxiter": kw.pop("maxiter", 100000), "check_every": kw.pop("check_every", 8)}
-    return out, kw
+    if kw:
+        raise ValueError("Unknown solver_kw: ", sorted(kw))
+    return out
 
 
 def solve_lin(K: StiffnessOperator, rhs, solver: str, rhs0=None, solver_kw: Dict = {},
               factorization=None, preconditioner: Optional[str] = None,
-              preconditioner_kw: Dict = {}, P=None, logger=None, **kwargs: Any):
+              preconditioner_kw: Dict = {}, P=None, logger=EmptyLogger, **kwargs: Any):
     if solver not in SOLVERS:
         raise ValueError("Unknown solver: ", solver)
     if not isinstance(K, StiffnessOperator):
@@ -47,14 +60,15 @@ def solve_lin(K: StiffnessOperator, rhs, solver: str, rhs0=None, solver_kw: Dict
         raise ValueError(f"rhs has {rhs.shape[0]} rows, K has {nfree} free DOFs")
     x0 = None if rhs0 is None else torch.as_tensor(rhs0, dtype=torch.float64,
                                                    device=dev).reshape(nfree, -1)
-    opts, rest = _solver_opts(solver_kw)
+    opts = _solver_opts(solver_kw)
     if solver == "amd-gmgpcg":
         prec = "gmg"
+        mg_kw = {k: v for k, v in preconditioner_kw.items() if k != "name"}
+        unknown = sorted(set(mg_kw) - set(MG_KEYS))
+        if unknown:
+            raise ValueError("Unknown preconditioner_kw: ", unknown)
         if P is None or K._mg_key is None:
-            mg_kw = dict(preconditioner_kw)
-            K.setup_mg(**{k: v for k, v in mg_kw.items()
-                          if k in ("nlevels", "omega", "nu1", "nu2", "precision",
-                                   "max_coarse_dof")})
+            K.setup_mg(**mg_kw)
         P = K
     else:
         prec = {None: "none", "amd-jacobi": "jacobi", "jacobi": "jacobi",
@@ -73,7 +87,7 @@ def solve_lin(K: StiffnessOperator, rhs, solver: str, rhs0=None, solver_kw: Dict
         if info["status"] != 0:
             raise RuntimeError("pcg iteration did not converge for bc ", i)
         sol[:, i] = u[K._free_t]
-        if logger is not None and hasattr(logger, "perf"):
+        if not isinstance(logger, type):  # the default is the EmptyLogger class itself
             logger.perf(f"PCGit. {info['iterations']}")
     K.last_infos = infos
     return sol, None, P

@@ -4,18 +4,21 @@
 Supported branch (the hot path): compliance objective, modified SIMP,
 ``optimizer="oc"`` (oc_top88), ``ft`` in {-1, 0, 1} with ``filter_mode`` in
 {"matrix", "convolution"} (both are the same distance filter), structured
-Q4/H8 grids with unit elements, no body forces, springs or element flags.
-Other options raise ``NotImplementedError``.
+Q4/H8 grids with unit elements, no body forces or springs; element flags
+(passive 1 / active 2) with the OC optimizer.  Other options raise
+``NotImplementedError``.
 
 Per iteration everything stays on the device: SIMP scale, matrix-free
 GMG-PCG solve per load case (warm started from the previous displacement,
 ``rhs0 = u[free]`` at topology_optimization.py:455), fused compliance +
 sensitivity kernel, filter stencils and the device OC bisection; only the
-scalars of the log line cross to the host.
+scalars of the log line cross to the host.  With ``output_kw["write_log"]``
+the header of topology_optimization.py:180-199 and one line per iteration
+(669-670) are written by ``SimpleLogger`` (log_utils.py:147-350), so
+``parse_simple_logfile`` reads the file back.
 """
 from __future__ import annotations
 
-import logging
 from typing import Callable, Dict, List, Optional
 
 import numpy as np
@@ -25,9 +28,13 @@ from .elements import lk_linear_elast_2d, lk_linear_elast_3d
 from .example_bc import mbb_2d
 from .fem import StiffnessOperator
 from .filter import DeviceFilter, HelmholtzFilter
+from .log_utils import EmptyLogger, SimpleLogger
 from .optimizer import mean_maxdiff, mma_defaultkws, mma_update, oc_top88
 
 LOG_LINE = "it.: {0} obj.: {1:.10f} vol.: {2:.10f} ch.: {3:.10f}"
+FILTER_NAMES = ["Sensitivity based", "Density based", "Haeviside Guest",
+                "Haeviside complement Sigmund 2007", "Haeviside eta projection",
+                "Volume Preserving eta projection", "No filter"]
 
 
 def default_outputkw() -> Dict:
@@ -58,8 +65,12 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
     if filter_mode not in ("matrix", "convolution", "helmholtz"):
         raise NotImplementedError(f"filter_mode {filter_mode!r} is not on the device path")
     helm = filter_mode == "helmholtz" and ft in (0, 1)
-    if body_forces_kw or el_flags is not None or mix is not None:
-        raise NotImplementedError("body forces, element flags and mixing are not supported")
+    if body_forces_kw or mix is not None:
+        raise NotImplementedError("body forces and mixing are not supported")
+    if el_flags is not None and optimizer != "oc":
+        # topology_optimization.py:248-259 gives passive elements x = 1 with
+        # bounds [0, 1e-9] under MMA, a subproblem with empty boxes
+        raise NotImplementedError("element flags are supported with optimizer='oc'")
     if (accelerator_kw or {}).get("accelerator") is not None:
         raise NotImplementedError("accelerators are not supported")
     if simulation_kw.get("grid", "regular") != "regular":
@@ -89,12 +100,17 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
     if springs:
         raise NotImplementedError("springs are not supported")
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
-    log = None
-    if output_kw.get("write_log"):
-        log = logging.getLogger("topoptlab_amd")
-        if output_kw.get("file"):
-            log.addHandler(logging.FileHandler(output_kw["file"] + ".log", mode="w"))
-        log.setLevel(output_kw.get("verbosity", 20))
+    out_kw = dict(default_outputkw(), **output_kw)
+    if out_kw["write_log"]:  # topology_optimization.py:180-199
+        log = SimpleLogger(file=out_kw["file"], verbosity=out_kw["verbosity"])
+        log.info(f"optimizer {optimizer}")
+        log.info(f"number of spatial dimensions: {ndim}")
+        log.info("elements: " + " x ".join([f"{nelx}", f"{nely}", f"{nelz}"][:ndim]))
+        log.info(f"volfrac: {volfrac} rmin: {rmin}" if volfrac is not None else f"rmin: {rmin}")
+        log.info("filter: " + FILTER_NAMES[ft])
+        log.info(f"filter mode: {filter_mode}")
+    else:
+        log = EmptyLogger()
     op = StiffnessOperator(nelx, nely, nelz, KE, fixed, eps, penal, device=dev)
     if helm:  # topology_optimization.py:360-365
         H = HelmholtzFilter(nelx, nely, rmin, nelz, device=dev)
@@ -106,6 +122,11 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
     x = torch.full((n,), float(volfrac), dtype=torch.float64, device=dev)
     xPhys = x.clone()
     g = 0.0
+    flags = None
+    if el_flags is not None:
+        flags = torch.as_tensor(np.asarray(el_flags).reshape(-1), dtype=torch.int32, device=dev)
+        if flags.numel() != n:
+            raise ValueError("el_flags must have one entry per element")
     mma = optimizer == "mma"
     if mma:  # topology_optimization.py:244-246; mma_utils.py:8-60
         mkw = mma_defaultkws(n, 1, ft, device=dev)
@@ -146,7 +167,7 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
             x = mma_update(x, xhist[-1], xhist[-2], dc, fval, dv_f, mkw, iter=ncases - 1)
             xhist = (xhist + [x.clone()])[-4:]
         else:
-            x, g = oc_top88(x, volfrac, dc, dv_f, g)
+            x, g = oc_top88(x, volfrac, dc, dv_f, g, el_flags=flags)
         if helm:  # 646-647
             xPhys = H.apply_filter(x) if ft == 1 else x.clone()
         else:
@@ -154,13 +175,12 @@ def main(nelx: int, nely: int, volfrac: float, rmin: float, ft: int = 1,
         vol, change = mean_maxdiff(xPhys, None)[0], mean_maxdiff(x, xold)[1]
         if history is not None:
             history.append((obj, vol, change))
-        if log is not None:
-            log.info(LOG_LINE.format(loop + 1, obj, vol, change))
+        log.info(LOG_LINE.format(loop + 1, obj, vol, change))
         if stop_on_change and change < 0.01:
             break
-    if output_kw.get("export"):  # topology_optimization.py:675-679
+    if out_kw["export"]:  # topology_optimization.py:675-679
         from .output_designs import export_vtk
-        export_vtk(output_kw["file"], nelx, nely, xPhys.cpu().numpy(), nelz,
+        export_vtk(out_kw["file"], nelx, nely, xPhys.cpu().numpy(), nelz,
                    x=x.cpu().numpy(), u=u.T.cpu().numpy(), f=f.T.cpu().numpy(), volfrac=volfrac)
     op.close()
     if H is not None:
