@@ -175,7 +175,10 @@ def test_c2_boundary_conditions_against_direct_solve_fixture(gpu):
         u_err = np.abs(ud[idx] - fx["u_idx"]).max() / scale
         print(f"{vc} gamma {gamma} rtol {rtol}: {info['iterations']} its, compliance err "
               f"{c_err:.2e}, sampled u err {u_err:.2e}", flush=True)
-        assert info["relres_true"] < 2 * rtol
+        # PCG stops on the recursive residual; under the 1e9 stiffness contrast
+        # the recomputed true residual stalls near 1e-11 (the direct solve's
+        # own is 3e-11)
+        assert info["relres_true"] < max(2 * rtol, 1e-10)
         assert c_err < ctol
         assert u_err < utol
     op.close()
