@@ -175,10 +175,27 @@ int tl_helmholtz_transfer(int32_t nelx, int32_t nely, const double* in, double* 
 /* Optimality-criteria bisection of optimizer/optimality_criterion.py:9-79.
  * x, dc, dv, xnew: device n; g, gt: host scalars.  el_flags (device n int32,
  * may be NULL): 1 = passive element (xnew 0), 2 = active element (xnew 1),
- * applied inside every bisection step as at optimality_criterion.py:68-71. */
+ * applied inside every bisection step as at optimality_criterion.py:68-71.
+ * The reduction scratch is allocated on the first call per device and
+ * shared; concurrent host threads are serialised by a lock. */
 int tl_oc_update(int64_t n, const double* x, const double* dc,
                  const double* dv, const int32_t* el_flags, double g, double move,
                  double l1, double l2, double* xnew, double* gt, void* stream);
+
+/* tl_oc_update without any device allocation.  No counterpart in topoptlab:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and results as tl_oc_update; the reduction partials and
+ * the bisection state live in a statically allocated device array of
+ * 41,216 bytes per device (held until the process ends), whose size does not
+ * depend on n (the grid is capped at 1024 blocks), so no n is too large for
+ * it.  Its address is looked up once per device on the first call.  Every
+ * call on a device shares that device's array: calls that overlap on one
+ * device (several host threads or streams) race and must be serialised by
+ * the caller. */
+int tl_oc_update_noalloc(int64_t n, const double* x, const double* dc,
+                         const double* dv, const int32_t* el_flags, double g, double move,
+                         double l1, double l2, double* xnew, double* gt, void* stream);
 
 /* Instrumentation (no counterpart in the reference): while enabled, every
  * launch of the fine-grid operator kernel and of the first coarse level's

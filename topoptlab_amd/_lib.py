@@ -85,6 +85,10 @@ _SIGS = {
                                C.c_double,
                                C.c_double, C.c_double, C.c_double, C.c_void_p,
                                C.POINTER(C.c_double), C.c_void_p]),
+    "tl_oc_update_noalloc": (C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_double, C.c_double, C.c_double,
+                                       C.c_double, C.c_void_p, C.POINTER(C.c_double),
+                                       C.c_void_p]),
     "tl_profile": (C.c_int, [C.c_void_p, C.c_int32]),
     "tl_profile_read": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(C.c_double),
                                   C.POINTER(C.c_int64)]),

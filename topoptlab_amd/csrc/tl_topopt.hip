@@ -328,6 +328,47 @@ int oc_bisect(int64_t n, const double* x, const double* dc, const double* dv,
   return TL_OK;
 }
 
+// Scratch of tl_oc_update_noalloc, statically allocated on every device the
+// library runs on: the block partials of oc_step_kernel (two values per
+// block at stride kMaxRedBlocks, at most kOcBlocks blocks) followed by the
+// bisection state.  41,216 bytes per device, held until the process ends.
+constexpr size_t kNoAllocPartialDoubles = size_t(kMaxRedBlocks) + kOcBlocks;
+constexpr size_t kNoAllocStateOffset = sizeof(double) * kNoAllocPartialDoubles;
+constexpr size_t kNoAllocScratchBytes = kNoAllocStateOffset + 256;
+static_assert(sizeof(OcState) <= 256, "bisection state must fit its scratch slot");
+static_assert(kNoAllocStateOffset % 256 == 0, "state slot must stay 256-byte aligned");
+static_assert(kOcBlocks <= kMaxRedBlocks, "oc_bisect's grid must fit the partial stride");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+// Device address of `scratch` per device, looked up once with that device
+// current (the symbol names a different copy on every device).
+struct NoAllocScratch {
+  std::mutex mu;
+  void* ptr[64] = {};
+};
+
+NoAllocScratch& noalloc_scratch() {
+  static NoAllocScratch s;
+  return s;
+}
+
+int noalloc_scratch_ptr(void** out) {
+  int dev = 0;
+  TL_HIP(hipGetDevice(&dev));
+  TL_ARG(dev >= 0 && dev < 64, "device ordinal out of range for the no-alloc scratch table");
+  NoAllocScratch& S = noalloc_scratch();
+  std::lock_guard<std::mutex> lock(S.mu);
+  if (!S.ptr[dev]) {
+    void* p = nullptr;
+    TL_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    TL_ARG(p, "hipGetSymbolAddress returned a null scratch address");
+    S.ptr[dev] = p;
+  }
+  *out = S.ptr[dev];
+  return TL_OK;
+}
+
 }  // namespace
 }  // namespace tl
 
@@ -494,6 +535,19 @@ int tl_oc_update(int64_t n, const double* x, const double* dc, const double* dv,
   TL_TRY(S.ensure());
   return oc_bisect(n, x, dc, dv, el_flags, g, move, l1, l2, xnew, gt,
                    static_cast<hipStream_t>(stream), S.partials.as<double>(), S.st.as<OcState>());
+}
+
+int tl_oc_update_noalloc(int64_t n, const double* x, const double* dc, const double* dv,
+                         const int32_t* el_flags, double g, double move, double l1, double l2,
+                         double* xnew, double* gt, void* stream) {
+  TL_ARG(x && dc && dv && xnew && gt, "null argument");
+  TL_ARG(n > 0, "n must be positive");
+  void* base = nullptr;
+  TL_TRY(noalloc_scratch_ptr(&base));
+  auto* bytes = static_cast<unsigned char*>(base);
+  return oc_bisect(n, x, dc, dv, el_flags, g, move, l1, l2, xnew, gt,
+                   static_cast<hipStream_t>(stream), reinterpret_cast<double*>(bytes),
+                   reinterpret_cast<OcState*>(bytes + kNoAllocStateOffset));
 }
 
 int tl_mean_maxdiff(int64_t n, const double* a, const double* b, double* mean, double* maxdiff,
