@@ -354,21 +354,14 @@ __device__ __forceinline__ void st_term(const GridD& g, const T* __restrict__ st
 // PRO (MODE 1, DIMU 2): x is the iterate before the coarse correction cc,
 // which is added on the fly (prolongation and first post-smoothing sweep in
 // one pass, with the values the two separate passes give).
+// The row of node n (n < g.nnodes): one thread, all NDPN DOFs.
 template <int NDPN, typename T, int MODE, int DIMU, bool PRO = false>
-__global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restrict__ sth,
-                                                      const T* __restrict__ x,
-                                                      const T* __restrict__ b,
-                                                      const T* __restrict__ dinv, T omega,
-                                                      const uint8_t* __restrict__ mask,
-                                                      T* __restrict__ y, const int* done,
-                                                      CoarseCorr<T> cc) {
-  static_assert(!PRO || (MODE == 1 && DIMU == 2), "fused correction: unrolled 2D Jacobi sweep");
-  if (*done) return;
-  const unsigned nb = gridDim.x, hw = blockIdx.x;
-  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
-  const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
-  const int64_t n = int64_t(logical) * BS + threadIdx.x;
-  if (n >= g.nnodes) return;
+__device__ __forceinline__ void st_apply_node(const GridD& g, const T* __restrict__ sth,
+                                              const T* __restrict__ x, const T* __restrict__ b,
+                                              const T* __restrict__ dinv, T omega,
+                                              const uint8_t* __restrict__ mask,
+                                              T* __restrict__ y, int64_t n,
+                                              const CoarseCorr<T>& cc = CoarseCorr<T>{}) {
   int ix, iy, iz;
   node_xyz(g, n, ix, iy, iz);
   T acc[NDPN];
@@ -406,6 +399,24 @@ __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restri
   }
 }
 
+template <int NDPN, typename T, int MODE, int DIMU, bool PRO = false>
+__global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restrict__ sth,
+                                                      const T* __restrict__ x,
+                                                      const T* __restrict__ b,
+                                                      const T* __restrict__ dinv, T omega,
+                                                      const uint8_t* __restrict__ mask,
+                                                      T* __restrict__ y, const int* done,
+                                                      CoarseCorr<T> cc) {
+  static_assert(!PRO || (MODE == 1 && DIMU == 2), "fused correction: unrolled 2D Jacobi sweep");
+  if (*done) return;
+  const unsigned nb = gridDim.x, hw = blockIdx.x;
+  const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
+  const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
+  const int64_t n = int64_t(logical) * BS + threadIdx.x;
+  if (n >= g.nnodes) return;
+  st_apply_node<NDPN, T, MODE, DIMU, PRO>(g, sth, x, b, dinv, omega, mask, y, n, cc);
+}
+
 // x = omega*dinv*b  (first smoothing sweep from a zero guess); b may be fp64
 template <typename T, typename TB>
 __global__ void jacobi_zero_kernel(const TB* __restrict__ b, const T* __restrict__ dinv, T omega,
@@ -423,13 +434,11 @@ __global__ void jacobi_zero_kernel(const TB* __restrict__ b, const T* __restrict
 // level's first smoothing sweep from the zero guess, x = omega D^-1 b, is
 // written in the same pass (the coarse cycle then starts from it).
 template <int NDPN, typename T, bool D3, bool SEED>
-__global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
-                                const uint8_t* __restrict__ maskc, T* __restrict__ bc,
-                                const T* __restrict__ dinvc, T omega, T* __restrict__ xc,
-                                const int* done) {
-  if (*done) return;
-  const int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (n >= gc.nnodes) return;
+__device__ __forceinline__ void restrict_node(const GridD& gf, const GridD& gc,
+                                              const T* __restrict__ rf,
+                                              const uint8_t* __restrict__ maskc,
+                                              T* __restrict__ bc, const T* __restrict__ dinvc,
+                                              T omega, T* __restrict__ xc, int64_t n) {
   int ix, iy, iz;
   node_xyz(gc, n, ix, iy, iz);
   T acc[NDPN];
@@ -460,14 +469,23 @@ __global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
   }
 }
 
-// fine x += P xc (masked)
-template <int NDPN, typename T, bool D3>
-__global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
-                                   const uint8_t* __restrict__ maskf, T* __restrict__ xf,
-                                   const int* done) {
+template <int NDPN, typename T, bool D3, bool SEED>
+__global__ void restrict_kernel(GridD gf, GridD gc, const T* __restrict__ rf,
+                                const uint8_t* __restrict__ maskc, T* __restrict__ bc,
+                                const T* __restrict__ dinvc, T omega, T* __restrict__ xc,
+                                const int* done) {
   if (*done) return;
   const int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
-  if (n >= gf.nnodes) return;
+  if (n >= gc.nnodes) return;
+  restrict_node<NDPN, T, D3, SEED>(gf, gc, rf, maskc, bc, dinvc, omega, xc, n);
+}
+
+// fine x += P xc (masked), node n of the fine grid
+template <int NDPN, typename T, bool D3>
+__device__ __forceinline__ void prolong_node(const GridD& gf, const GridD& gc,
+                                             const T* __restrict__ xc,
+                                             const uint8_t* __restrict__ maskf,
+                                             T* __restrict__ xf, int64_t n) {
   int ix, iy, iz;
   node_xyz(gf, n, ix, iy, iz);
   T acc[NDPN];
@@ -476,6 +494,16 @@ __global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
 #pragma unroll
   for (int d = 0; d < NDPN; ++d)
     if (!((mk >> d) & 1)) xf[n * NDPN + d] += acc[d];
+}
+
+template <int NDPN, typename T, bool D3>
+__global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
+                                   const uint8_t* __restrict__ maskf, T* __restrict__ xf,
+                                   const int* done) {
+  if (*done) return;
+  const int64_t n = blockIdx.x * int64_t(blockDim.x) + threadIdx.x;
+  if (n >= gf.nnodes) return;
+  prolong_node<NDPN, T, D3>(gf, gc, xc, maskf, xf, n);
 }
 
 template <int NDPN, typename T>
@@ -510,15 +538,12 @@ int launch_prolong(tl_problem* P, const Grid& gf, const Grid& gc, const T* xc,
   return TL_OK;
 }
 
-// x = Cinv b on the coarsest grid (Cinv fp64, padded leading dimension)
+// x[row] = (Cinv b)[row], one wavefront per row (Cinv fp64, padded leading
+// dimension np)
 template <typename T>
-__global__ __launch_bounds__(BS) void dense_gemv_kernel(const double* __restrict__ A, int64_t np,
-                                                        int64_t n, const T* __restrict__ b,
-                                                        T* __restrict__ x, const int* done) {
-  if (*done) return;
-  const int64_t row = blockIdx.x * int64_t(BS / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (row >= n) return;
+__device__ __forceinline__ void gemv_row(const double* __restrict__ A, int64_t np, int64_t n,
+                                         const T* __restrict__ b, T* __restrict__ x, int64_t row,
+                                         int lane) {
   const double* __restrict__ Ar = A + row * np;
   double s = 0.0;
   // U loads per lane in flight before the first FMA (a plain loop waited for
@@ -539,6 +564,17 @@ __global__ __launch_bounds__(BS) void dense_gemv_kernel(const double* __restrict
   for (; j < n; j += 64) s += Ar[j] * double(b[j]);
   s = wave_sum(s);
   if (lane == 0) x[row] = T(s);
+}
+
+// x = Cinv b on the coarsest grid
+template <typename T>
+__global__ __launch_bounds__(BS) void dense_gemv_kernel(const double* __restrict__ A, int64_t np,
+                                                        int64_t n, const T* __restrict__ b,
+                                                        T* __restrict__ x, const int* done) {
+  if (*done) return;
+  const int64_t row = blockIdx.x * int64_t(BS / 64) + (threadIdx.x >> 6);
+  if (row >= n) return;
+  gemv_row<T>(A, np, n, b, x, row, threadIdx.x & 63);
 }
 
 // ---------------------------------------------------------------------------
