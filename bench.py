@@ -250,8 +250,9 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True):
             op.set_density(x)
             op.setup_mg(nlevels=lv, precision="fp32")
 
-    def solve_all(graphs=True, streams=nstreams):
-        return batch_solve(ops, fs, rtol=1e-8, nstreams=streams, chunk=chunk, use_graphs=graphs)
+    def solve_all(graphs=True, streams=nstreams, lockstep=True):
+        return batch_solve(ops, fs, rtol=1e-8, nstreams=streams, chunk=chunk, use_graphs=graphs,
+                           lockstep=lockstep)
 
     def sequential():
         for op, f in zip(ops, fs):
@@ -268,13 +269,14 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True):
 
     setup_all()
     tiles = [autotune(op, reps=10) for op in ops]
-    _, infos = solve_all()  # warm-up: captures every instance's chunk graph
+    _, infos = solve_all()  # warm-up
     ms_step = timed(lambda: (setup_all(), solve_all()), steps)
     ms_setup = timed(setup_all, steps)
-    solve_all()  # the new hierarchies invalidate the captured graphs: capture again
-    ms_solve = timed(solve_all, steps)  # graphs already captured for this hierarchy
-    ms_eager = timed(lambda: solve_all(graphs=False), steps)
-    ms_one_stream = timed(lambda: solve_all(streams=1), steps)
+    ms_solve = timed(solve_all, steps)  # lockstep: all systems in every launch
+    solve_all(lockstep=False)  # captures every instance's chunk graph for this hierarchy
+    ms_streams = timed(lambda: solve_all(lockstep=False), steps)
+    ms_eager = timed(lambda: solve_all(graphs=False, lockstep=False), steps)
+    ms_one_stream = timed(lambda: solve_all(streams=1, lockstep=False), steps)
     ms_seq = timed(sequential, 1)
     its = [i["iterations"] for lst in infos for i in lst]
     rr = [i["relres_true"] for lst in infos for i in lst]
@@ -282,18 +284,22 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True):
     hist = {}
     for t in tiles:
         hist[str(t["fp64"])] = hist.get(str(t["fp64"]), 0) + 1
-    log(f"batched: step {ms_step:.1f} ms (setup {ms_setup:.1f}, batch solve {ms_solve:.1f}, "
-        f"eager {ms_eager:.1f}, one stream {ms_one_stream:.1f}, one at a time {ms_seq:.1f}); "
+    log(f"batched: step {ms_step:.1f} ms (setup {ms_setup:.1f}, lockstep batch solve "
+        f"{ms_solve:.1f}, per-instance graphs on {nstreams} streams {ms_streams:.1f}, eager "
+        f"{ms_eager:.1f}, one stream {ms_one_stream:.1f}, one at a time {ms_seq:.1f}); "
         f"iterations {min(its)}..{max(its)}, max true relres {max(rr):.2e}")
     out = {
         "workload": "32 Q4 elasticity grids (nelx 128..1024, nely 64..512, multiples of 32, "
                     "4 point-load cases, left edge clamped, Emin 1e-9) + 8 Q4 heat grids "
                     "(kmin 1e-3, uniform source, left edge T=0); GRF densities seed 0; "
-                    "FP64 PCG, FP32 GMG V(1,1) (coarsest grid <= 3000 DOF), relres 1e-8",
+                    "FP64 PCG, FP32 GMG V(1,1) (coarsest grid <= 3000 DOF), relres 1e-8; "
+                    "all 136 systems in lockstep, one launch per kernel and place in the cycle",
         "instances": len(ops), "systems": len(its), "dof_total": int(sum(op.ndof for op in ops)),
         "dof_rhs_total": int(dof_rhs),
         "ms_per_step": round(ms_step, 2), "ms_setup": round(ms_setup, 2),
-        "ms_batch_solve": round(ms_solve, 2), "ms_batch_solve_eager": round(ms_eager, 2),
+        "ms_batch_solve": round(ms_solve, 2), "batch_solve_mode": "lockstep",
+        "ms_batch_solve_stream_graphs": round(ms_streams, 2),
+        "ms_batch_solve_eager": round(ms_eager, 2),
         "ms_batch_solve_one_stream": round(ms_one_stream, 2),
         "ms_one_at_a_time": round(ms_seq, 2),
         "speedup_batch_vs_one_at_a_time": round(ms_seq / ms_solve, 2),

@@ -124,15 +124,20 @@ def test_batch_matches_single_solves(gpu):
         xs.append(x)
         fs.append(torch.as_tensor(f, device=gpu))
     ref = [[op.solve(f[k], rtol=1e-8) for k in range(f.shape[0])] for op, f in zip(ops, fs)]
-    for graphs in (True, False, True):
-        say("batch solve, graphs", graphs)
-        U, infos = batch_solve(ops, fs, rtol=1e-8, nstreams=3, chunk=4, use_graphs=graphs)
+    # lockstep (one launch per kernel and place in the cycle for all systems),
+    # then per-instance streams with and without captured graphs; each twice
+    # in a row (the second call reuses what the first set up)
+    for lockstep, graphs in ((True, True), (True, True), (False, True), (False, False),
+                             (False, True)):
+        say("batch solve, lockstep", lockstep, "graphs", graphs)
+        U, infos = batch_solve(ops, fs, rtol=1e-8, nstreams=3, chunk=4, use_graphs=graphs,
+                               lockstep=lockstep)
         for i in range(len(ops)):
             assert U[i].shape == fs[i].shape
             for k in range(fs[i].shape[0]):
                 u_ref, inf_ref = ref[i][k]
-                assert infos[i][k]["iterations"] == inf_ref["iterations"], (i, k, graphs)
-                assert torch.equal(U[i][k], u_ref), (i, k, graphs)
+                assert infos[i][k]["iterations"] == inf_ref["iterations"], (i, k, lockstep, graphs)
+                assert torch.equal(U[i][k], u_ref), (i, k, lockstep, graphs)
                 # PCG stops on the recursive residual (linear_solvers.py:107-117).
                 # A point load on near-void material (Emin 1e-9) gives |u| up
                 # to ~1e7 and the recomputed residual sits above the recursive
@@ -173,11 +178,14 @@ def test_batch_reports_non_convergence(gpu):
         op.setup_mg(nlevels=levels, precision="fp32")
         ops.append(op)
         fs.append(torch.as_tensor(f, device=gpu))
-    with pytest.raises(RuntimeError):
-        batch_solve(ops, fs, rtol=1e-12, maxiter=2, chunk=2)
-    U, infos = batch_solve(ops, fs, rtol=1e-12, maxiter=2, chunk=2, raise_on_fail=False)
-    for lst in infos:
-        for inf in lst:
-            assert inf["status"] != 0 and inf["iterations"] <= 4
+    for lockstep in (True, False):
+        say("batch non-convergence, lockstep", lockstep)
+        with pytest.raises(RuntimeError):
+            batch_solve(ops, fs, rtol=1e-12, maxiter=2, chunk=2, lockstep=lockstep)
+        U, infos = batch_solve(ops, fs, rtol=1e-12, maxiter=2, chunk=2, raise_on_fail=False,
+                               lockstep=lockstep)
+        for lst in infos:
+            for inf in lst:
+                assert inf["status"] != 0 and inf["iterations"] <= 4
     for op in ops:
         op.close()

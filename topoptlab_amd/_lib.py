@@ -37,7 +37,8 @@ class SolveOpts(C.Structure):
 
 
 class BatchOpts(C.Structure):
-    _fields_ = [("nstreams", C.c_int32), ("chunk", C.c_int32), ("use_graphs", C.c_int32)]
+    _fields_ = [("nstreams", C.c_int32), ("chunk", C.c_int32), ("use_graphs", C.c_int32),
+                ("lockstep", C.c_int32)]
 
 
 class MmaOpts(C.Structure):

@@ -5,12 +5,19 @@ problem (``solve_linsystem.py:341-352``, ``topology_optimization.py:463-505``).
 Here a whole batch of :class:`~topoptlab_amd.fem.StiffnessOperator` instances
 of uneven grid sizes is handed to ``tl_batch_solve`` at once:
 
-* every instance runs the same FP64 PCG as ``StiffnessOperator.solve`` (same
-  kernels, same arithmetic, same stopping rule);
-* instances run concurrently on several HIP streams; each instance's chunk of
-  PCG iterations (V-cycle, fused product, update) is captured once into a
-  hipGraph and replayed until the device-side state reports convergence;
-* the load cases of one instance run one after another on its stream;
+* every system (instance and load case) runs the same FP64 PCG as
+  ``StiffnessOperator.solve`` (same kernels, same arithmetic, same stopping
+  rule), so each result equals the single solve bit for bit;
+* ``lockstep=True`` (2D instances with GMG V(1,1) cycles): all systems
+  iterate together, each with its own PCG and V-cycle vectors; one PCG
+  iteration of every system is recorded once, the recordings are merged, and
+  each iteration is one launch per kernel and place in the cycle covering the
+  blocks of every system (a tile table maps blocks to systems); converged
+  systems drop out of the launches;
+* otherwise instances run concurrently on several HIP streams; each
+  instance's chunk of PCG iterations is captured once into a hipGraph and
+  replayed until the device-side state reports convergence, and the load
+  cases of one instance run one after another on its stream;
 * ``autotune`` picks the 2D operator's workgroup shape per grid by timing.
 """
 from __future__ import annotations
@@ -53,7 +60,8 @@ def clear_autotune_table() -> None:
 def batch_solve(ops: Sequence[StiffnessOperator], fs: Sequence[torch.Tensor], rtol: float = 1e-8,
                 atol: float = 0.0, maxiter: int = 100000, preconditioner: str = "gmg",
                 nstreams: int = 16, chunk: int = 8, use_graphs: bool = True,
-                x0: Optional[Sequence[torch.Tensor]] = None, raise_on_fail: bool = True):
+                x0: Optional[Sequence[torch.Tensor]] = None, raise_on_fail: bool = True,
+                lockstep: bool = True):
     """Solve K_i u = f_i for every instance i; ``fs[i]`` holds one (ndof,) or
     several (nrhs, ndof) full-length float64 right-hand sides.  Returns the
     solutions (same shapes) and per instance a list of info dicts."""
@@ -86,7 +94,7 @@ def batch_solve(ops: Sequence[StiffnessOperator], fs: Sequence[torch.Tensor], rt
     infos = (_lib.SolveInfo * sum(nrhs))()
     opts = _lib.SolveOpts(float(rtol), float(atol), int(maxiter), int(x0 is not None),
                           PRECONDITIONERS[preconditioner], int(chunk))
-    bo = _lib.BatchOpts(int(nstreams), int(chunk), int(bool(use_graphs)))
+    bo = _lib.BatchOpts(int(nstreams), int(chunk), int(bool(use_graphs)), int(bool(lockstep)))
     rc = lib().tl_batch_solve(n, handles, fptr, uptr, nr, C.byref(opts), C.byref(bo), infos)
     if rc != _lib.TL_ENOCONV or raise_on_fail:
         check(rc, "tl_batch_solve")

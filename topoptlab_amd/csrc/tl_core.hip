@@ -584,11 +584,11 @@ __global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
 }
 
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
-__global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
+__device__ __forceinline__ void ku2d_body(const KuArgs& a, unsigned bid) {
   if (a.done && *a.done) return;
   extern __shared__ double lds_raw[];
   T* lds = reinterpret_cast<T*>(lds_raw);
-  const int64_t j0 = int64_t(blockIdx.x) * BS;
+  const int64_t j0 = int64_t(bid) * BS;
   const int64_t j = j0 + threadIdx.x;
   const bool active = j < a.plane;
   const int64_t jc = active ? j : a.plane - 1;
@@ -624,8 +624,13 @@ __global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
     __shared__ double sh[BS / 64 + 2];
     double v[1] = {dot};
     block_sum<1>(v, sh);
-    store_partials<1>(v, a.partials);
+    store_partials_at<1>(v, a.partials, bid);
   }
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
+__global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
+  ku2d_body<NDPN, T, MODE, TB, TO, FUSEP, FAST>(a, blockIdx.x);
 }
 
 static int launch_fold(tl_problem* P, unsigned nb, int fin, double* out, const int* done);
@@ -637,14 +642,14 @@ static int launch_fold(tl_problem* P, unsigned nb, int fin, double* out, const i
 // does not, but its rows are shorter.  tl_autotune picks the faster shape
 // per grid.
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
-__global__ __launch_bounds__(BS) void ku2t_kernel(KuArgs a) {
+__device__ __forceinline__ void ku2t_body(const KuArgs& a, unsigned bid) {
   if (a.done && *a.done) return;
   constexpr int TX = BS / TY;
   constexpr int HY = TY + 2, HN = (TX + 2) * HY;
   static_assert(TX * TY == BS, "tile must hold BS nodes");
   __shared__ T lds[HN * NDPN];
   const int ntx = (a.nx + TX - 1) / TX;
-  const int x0 = int(blockIdx.x % unsigned(ntx)) * TX, y0 = int(blockIdx.x / unsigned(ntx)) * TY;
+  const int x0 = int(bid % unsigned(ntx)) * TX, y0 = int(bid / unsigned(ntx)) * TY;
   const int lx = threadIdx.x / TY, ly = threadIdx.x % TY;
   const int ix = x0 + lx, iy = y0 + ly;
   const bool active = ix < a.nx && iy < a.ny;
@@ -694,8 +699,47 @@ __global__ __launch_bounds__(BS) void ku2t_kernel(KuArgs a) {
     __shared__ double sh[BS / 64 + 2];
     double v[1] = {dot};
     block_sum<1>(v, sh);
-    store_partials<1>(v, a.partials);
+    store_partials_at<1>(v, a.partials, bid);
   }
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
+__global__ __launch_bounds__(BS) void ku2t_kernel(KuArgs a) {
+  ku2t_body<NDPN, T, MODE, TB, TO, FUSEP, TY>(a, blockIdx.x);
+}
+
+// Multi-instance forms of the 2D operator for lockstep batched solves (see
+// tl::Recorder): block b runs local block tiles[b].y of system tiles[b].x.
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
+__global__ __launch_bounds__(BS) void ku2t_multi_kernel(const KuArgs* __restrict__ descs,
+                                                        const uint2* __restrict__ tiles) {
+  const uint2 t = tiles[blockIdx.x];
+  ku2t_body<NDPN, T, MODE, TB, TO, FUSEP, TY>(descs[t.x], t.y);
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
+static int ku2t_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                             size_t lds) {
+  hipLaunchKernelGGL((ku2t_multi_kernel<NDPN, T, MODE, TB, TO, FUSEP, TY>), dim3(nb), dim3(BS),
+                     lds, s, static_cast<const KuArgs*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
+__global__ __launch_bounds__(BS) void ku2d_multi_kernel(const KuArgs* __restrict__ descs,
+                                                        const uint2* __restrict__ tiles) {
+  const uint2 t = tiles[blockIdx.x];
+  ku2d_body<NDPN, T, MODE, TB, TO, FUSEP, FAST>(descs[t.x], t.y);
+}
+
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
+static int ku2d_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                             size_t lds) {
+  hipLaunchKernelGGL((ku2d_multi_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), dim3(nb), dim3(BS),
+                     lds, s, static_cast<const KuArgs*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
 }
 
 // Workgroups of BS threads that the current device holds at once for
@@ -732,9 +776,14 @@ static int launch_ku2t(const tl_problem* P, const KuArgs& a) {
     set_error("fine operator grid exceeds the reduction slot count");
     return TL_EINVAL;
   }
-  hipLaunchKernelGGL((ku2t_kernel<NDPN, T, MODE, TB, TO, FUSEP, TY>), dim3(unsigned(nb)), dim3(BS),
-                     0, P->stream, a);
-  TL_CHECK_LAUNCH();
+  if (P->rec) {
+    P->rec->add(P->rec_phase, P->rec_level, &ku2t_multi_launch<NDPN, T, MODE, TB, TO, FUSEP, TY>,
+                a, unsigned(nb), 0);
+  } else {
+    hipLaunchKernelGGL((ku2t_kernel<NDPN, T, MODE, TB, TO, FUSEP, TY>), dim3(unsigned(nb)),
+                       dim3(BS), 0, P->stream, a);
+    TL_CHECK_LAUNCH();
+  }
   if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT)
     return launch_fold(const_cast<tl_problem*>(P), unsigned(nb), a.fin, a.dot_out, a.done);
   return TL_OK;
@@ -748,6 +797,10 @@ static int launch_ku_impl(const tl_problem* P, KuArgs a) {
   a.nx = g.nx; a.ny = g.ny; a.nz = g.nz; a.plane = g.nx * g.ny;
   const int L = BS + 2 * g.ny + 2;
   const unsigned nbx = unsigned((int64_t(a.plane) + BS - 1) / BS);
+  if (g.dim == 3 && P->rec) {  // no multi-instance form of the 3D operator
+    P->rec->unsupported = true;
+    return TL_OK;
+  }
   if (g.dim == 3) {
     // p = z + beta p is a separate pass before the 3D product (the fused
     // form needed a second copy stream and an in-LDS combine and was
@@ -796,8 +849,14 @@ static int launch_ku_impl(const tl_problem* P, KuArgs a) {
       set_error("fine operator grid exceeds the reduction slot count");
       return TL_EINVAL;
     }
-    hipLaunchKernelGGL((ku2d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), dim3(nbx), dim3(BS),
-                       size_t(L) * NDPN * sizeof(T), P->stream, a);
+    if (P->rec) {
+      P->rec->add(P->rec_phase, P->rec_level,
+                  &ku2d_multi_launch<NDPN, T, MODE, TB, TO, FUSEP, FAST>, a, nbx,
+                  size_t(L) * NDPN * sizeof(T));
+    } else {
+      hipLaunchKernelGGL((ku2d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), dim3(nbx), dim3(BS),
+                         size_t(L) * NDPN * sizeof(T), P->stream, a);
+    }
     if constexpr (MODE == KU_STORE_DOT || MODE == KU_JACOBI_DOT) {
       TL_CHECK_LAUNCH();
       return launch_fold(const_cast<tl_problem*>(P), nbx, a.fin, a.dot_out, a.done);
@@ -898,6 +957,11 @@ int prof_end(tl_problem* P, ProfEvent& ev) {
 
 static int fine_apply_dispatch(tl_problem* P, KuArgs& a, int prec, int mode, bool fusep, bool b64,
                                bool out64) {
+  if (P->rec && (prec == TL_PREC_BF16 || prec == TL_PREC_FP16 || (prec == TL_PREC_FP64 &&
+                                                                   P->op.mfma64))) {
+    P->rec->unsupported = true;  // no multi-instance form of the MFMA operators
+    return TL_OK;
+  }
   if (prec == TL_PREC_BF16 || prec == TL_PREC_FP16) {
     // V-cycle smoothing and residual on the matrix cores
     TL_ARG(b64 && !fusep && (mode == KU_RESID || mode == KU_JACOBI || mode == KU_JACOBI_DOT),
@@ -1022,6 +1086,34 @@ __global__ __launch_bounds__(BS) void fold_kernel(const double* __restrict__ par
   if (threadIdx.x == 0) finalize(fin, st, tot[0], out);
 }
 
+struct FoldArgs {
+  const double* partials;
+  unsigned nb;
+  int fin;
+  PcgState* st;
+  double* out;
+  const int* done;
+};
+
+// one block per system (tiles[b].y == 0)
+__global__ __launch_bounds__(BS) void fold_multi_kernel(const FoldArgs* __restrict__ descs,
+                                                        const uint2* __restrict__ tiles) {
+  const FoldArgs a = descs[tiles[blockIdx.x].x];
+  if (a.done && *a.done) return;
+  __shared__ double sh[BS / 64 + 2];
+  double tot[1];
+  fold_partials<1>(a.partials, a.nb, tot, sh);
+  if (threadIdx.x == 0) finalize(a.fin, a.st, tot[0], a.out);
+}
+
+static int fold_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                             size_t lds) {
+  hipLaunchKernelGGL(fold_multi_kernel, dim3(nb), dim3(BS), lds, s,
+                     static_cast<const FoldArgs*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
 static unsigned red_blocks(int64_t n) {
   // enough workgroups to keep HBM busy (each thread streams ~5 elements at
   // the 5.4M-DOF size), bounded by the reduction slot count
@@ -1063,25 +1155,58 @@ __global__ __launch_bounds__(BS) void resid_kernel(const double* __restrict__ b,
 // also the next V-cycle's first fine-level sweep xs = omega * dinv * r from
 // the new residual, in the V-cycle's precision TS (saves that sweep's pass).
 template <typename TS>
-__global__ __launch_bounds__(BS) void update_kernel(double* __restrict__ x, double* __restrict__ r,
-                                                    const double* __restrict__ p,
-                                                    const double* __restrict__ q, int64_t n,
-                                                    double* partials, const PcgState* st,
-                                                    TS* __restrict__ xs,
-                                                    const TS* __restrict__ dinv, TS omega,
-                                                    const int* done) {
-  if (*done) return;
-  const double alpha = st->alpha;
+struct UpdateArgs {
+  double* x;
+  double* r;
+  const double* p;
+  const double* q;
+  int64_t n;
+  double* partials;
+  const PcgState* st;
+  TS* xs;
+  const TS* dinv;
+  TS omega;
+  const int* done;
+  unsigned nb;  // blocks of the system's grid-stride loop
+};
+
+template <typename TS>
+__device__ __forceinline__ void update_body(const UpdateArgs<TS>& a, unsigned bid) {
+  if (*a.done) return;
+  const double alpha = a.st->alpha;
   double s = 0.0;
-  for (int64_t i = blockIdx.x * int64_t(BS) + threadIdx.x; i < n; i += int64_t(gridDim.x) * BS) {
-    x[i] += alpha * p[i];
-    const double rv = r[i] - alpha * q[i];
-    r[i] = rv;
+  for (int64_t i = bid * int64_t(BS) + threadIdx.x; i < a.n; i += int64_t(a.nb) * BS) {
+    a.x[i] += alpha * a.p[i];
+    const double rv = a.r[i] - alpha * a.q[i];
+    a.r[i] = rv;
     s += rv * rv;
-    if (xs) xs[i] = zero_guess_sweep<TS, double>(omega, dinv[i], rv);
+    if (a.xs) a.xs[i] = zero_guess_sweep<TS, double>(a.omega, a.dinv[i], rv);
   }
   double v[1] = {s};
-  reduce_store<1>(v, partials);
+  __shared__ double sh[BS / 64 + 4];
+  block_sum<1>(v, sh);
+  store_partials_at<1>(v, a.partials, bid);
+}
+
+template <typename TS>
+__global__ __launch_bounds__(BS) void update_kernel(UpdateArgs<TS> a) {
+  update_body<TS>(a, blockIdx.x);
+}
+
+template <typename TS>
+__global__ __launch_bounds__(BS) void update_multi_kernel(const UpdateArgs<TS>* __restrict__ descs,
+                                                          const uint2* __restrict__ tiles) {
+  const uint2 t = tiles[blockIdx.x];
+  update_body<TS>(descs[t.x], t.y);
+}
+
+template <typename TS>
+static int update_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                               size_t lds) {
+  hipLaunchKernelGGL(update_multi_kernel<TS>, dim3(nb), dim3(BS), lds, s,
+                     static_cast<const UpdateArgs<TS>*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
 }
 
 // xs = omega * dinv * r (the first fine-level sweep of the first V-cycle)
@@ -1152,6 +1277,12 @@ static unsigned grid_for(int64_t n) {
 }
 
 static int launch_fold(tl_problem* P, unsigned nb, int fin, double* out, const int* done) {
+  if (P->rec) {
+    P->rec->add(P->rec_phase, P->rec_level, &fold_multi_launch,
+                FoldArgs{P->partials.as<double>(), nb, fin, P->state.as<PcgState>(), out, done},
+                1u, 0);
+    return TL_OK;
+  }
   hipLaunchKernelGGL(fold_kernel, dim3(1), dim3(BS), 0, P->stream, P->partials.as<double>(), nb,
                      fin, P->state.as<PcgState>(), out, done);
   TL_CHECK_LAUNCH();
@@ -1481,6 +1612,10 @@ int pcg_iteration(tl_problem* P, const PcgCtx& c, double* x, int precond, int64_
   if (precond == TL_PRECOND_GMG) {
     TL_TRY(mg_vcycle(P, c.r, c.z, c.done, fx0));
   } else {
+    if (P->rec) {  // no multi-instance form of the Jacobi preconditioner
+      P->rec->unsupported = true;
+      return TL_OK;
+    }
     hipLaunchKernelGGL(jacobi_prec_kernel, dim3(c.rb), dim3(BS), 0, s, c.r,
                        precond == TL_PRECOND_JACOBI ? P->op.dinv64.as<double>() : nullptr, c.z,
                        c.n, c.partials, c.done);
@@ -1492,6 +1627,12 @@ int pcg_iteration(tl_problem* P, const PcgCtx& c, double* x, int precond, int64_
   // two buffers.  The 3D vector operator takes the update as a separate pass.
   double* pin = c.pp[k & 1];
   double* pout = c.pp[(k + 1) & 1];
+  P->rec_phase = 2;  // the PCG update after the preconditioner
+  P->rec_level = 0;
+  if (P->g.dim == 3 && P->rec) {
+    P->rec->unsupported = true;
+    return TL_OK;
+  }
   if (P->g.dim == 3 && !(P->op.mfma64 && P->ndpn == 3)) {
     hipLaunchKernelGGL(pupdate_kernel, dim3(c.nb), dim3(BS), 0, s, c.z, pin, pout, &c.st->beta,
                        c.n, c.done);
@@ -1503,19 +1644,27 @@ int pcg_iteration(tl_problem* P, const PcgCtx& c, double* x, int precond, int64_
                          pin, pout, &c.st->beta, nullptr, FIN_PQ, c.done, false));
   }
   // x += alpha p ; r -= alpha q ; rr ; done
+  auto upd = [&](auto ua) -> int {
+    using TS = std::remove_pointer_t<decltype(ua.xs)>;
+    if (P->rec) {
+      P->rec->add(2, 0, &update_multi_launch<TS>, ua, c.rb, 0);
+      return TL_OK;
+    }
+    hipLaunchKernelGGL(update_kernel<TS>, dim3(c.rb), dim3(BS), 0, s, ua);
+    TL_CHECK_LAUNCH();
+    return TL_OK;
+  };
   if (fx0 && P->mg.prec == TL_PREC_FP64)
-    hipLaunchKernelGGL(update_kernel<double>, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n,
-                       c.partials, c.st, P->mg.lv[0].vx.as<double>(), P->op.dinv64.as<double>(),
-                       P->mg.omega, c.done);
+    TL_TRY(upd(UpdateArgs<double>{x, c.r, pout, c.q, c.n, c.partials, c.st,
+                                  P->mg.lv[0].vx.as<double>(), P->op.dinv64.as<double>(),
+                                  P->mg.omega, c.done, c.rb}));
   else if (fx0)
-    hipLaunchKernelGGL(update_kernel<float>, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n,
-                       c.partials, c.st, P->mg.lv[0].vx.as<float>(), P->op.dinv32.as<float>(),
-                       float(P->mg.omega), c.done);
+    TL_TRY(upd(UpdateArgs<float>{x, c.r, pout, c.q, c.n, c.partials, c.st,
+                                 P->mg.lv[0].vx.as<float>(), P->op.dinv32.as<float>(),
+                                 float(P->mg.omega), c.done, c.rb}));
   else
-    hipLaunchKernelGGL(update_kernel<double>, dim3(c.rb), dim3(BS), 0, s, x, c.r, pout, c.q, c.n,
-                       c.partials, c.st, static_cast<double*>(nullptr),
-                       static_cast<const double*>(nullptr), 0.0, c.done);
-  TL_CHECK_LAUNCH();
+    TL_TRY(upd(UpdateArgs<double>{x, c.r, pout, c.q, c.n, c.partials, c.st, nullptr, nullptr,
+                                  0.0, c.done, c.rb}));
   return launch_fold(P, c.rb, FIN_RR, nullptr, c.done);
 }
 
@@ -1744,6 +1893,294 @@ struct BatchSlot {
   double tr2;
   double pad[7];
 };
+
+// ---------------------------------------------------------------------------
+// Lockstep batched solves.  Every (instance, right-hand side) pair is a
+// system with its own PCG and V-cycle vectors (carved from one arena); the
+// hierarchy (stencils, masks, coarsest inverse) stays shared per instance.
+// One PCG iteration of each system is recorded once per parity of the
+// iteration counter (the search directions and the levels' iterate buffers
+// alternate), the recordings are merged across systems, and every iteration
+// replays the merged schedule: one launch per (place in the cycle, position,
+// kernel) covers the blocks of all systems that run that kernel there.
+// ---------------------------------------------------------------------------
+constexpr int kSysBufs = 9;  // r z p p2 q t xb partials state
+struct SysBufs {
+  DevBuf b[kSysBufs];
+  std::vector<DevBuf> lv;  // 4 per level: vb vx vr vt
+};
+
+DevBuf* sys_slot(tl_problem* P, int k) {
+  DevBuf* slots[kSysBufs] = {&P->r, &P->z, &P->p, &P->p2, &P->q, &P->t, &P->xb, &P->partials,
+                             &P->state};
+  return slots[k];
+}
+
+DevBuf* lv_slot(tl_problem* P, int l, int k) {
+  Level& L = P->mg.lv[l];
+  DevBuf* slots[4] = {&L.vb, &L.vx, &L.vr, &L.vt};
+  return slots[k];
+}
+
+// Exchange the handle's work vectors with the system's (an involution).
+void swap_sys(tl_problem* P, SysBufs& S) {
+  for (int k = 0; k < kSysBufs; ++k) std::swap(*sys_slot(P, k), S.b[k]);
+  for (int l = 0; l < P->mg.nlevels; ++l)
+    for (int k = 0; k < 4; ++k) std::swap(*lv_slot(P, l, k), S.lv[size_t(l) * 4 + k]);
+}
+
+// Byte sizes of a system's buffers on handle P (same as the handle's own).
+std::vector<size_t> sys_sizes(tl_problem* P) {
+  std::vector<size_t> sz;
+  const size_t v = sizeof(double) * size_t(P->ndof);
+  for (int k = 0; k < 7; ++k) sz.push_back(v);
+  sz.push_back(sizeof(double) * 4 * kMaxRedBlocks);
+  sz.push_back(sizeof(PcgState));
+  for (int l = 0; l < P->mg.nlevels; ++l)
+    for (int k = 0; k < 4; ++k) sz.push_back(std::max<size_t>(lv_slot(P, l, k)->bytes, 8));
+  return sz;
+}
+
+struct MergedLaunch {
+  MultiLaunchFn fn;
+  size_t lds = 0;
+  size_t desc_off = 0, tile_off = 0;  // byte offsets into the upload
+  unsigned nblocks = 0;
+};
+
+// Merge the recordings of the active systems into one launch list.  Places
+// run down the cycle by level, up the cycle by level, then the PCG update; at
+// each place the systems' ops run position by position, one launch per
+// kernel at each position, so every system's ops keep their order.
+void merge_schedule(const std::vector<const Recorder*>& recs, std::vector<MergedLaunch>& out,
+                    std::vector<unsigned char>& blob) {
+  out.clear();
+  blob.clear();
+  int maxl = 0;
+  for (const Recorder* r : recs)
+    for (const RecOp& op : r->ops) maxl = std::max(maxl, op.level);
+  std::vector<std::pair<int, int>> places;
+  for (int l = 0; l <= maxl; ++l) places.push_back({0, l});
+  for (int l = maxl; l >= 0; --l) places.push_back({1, l});
+  places.push_back({2, 0});
+  auto align = [&](size_t a) { blob.resize((blob.size() + a - 1) / a * a); };
+  for (const auto& pl : places) {
+    std::vector<std::vector<const RecOp*>> seq(recs.size());
+    size_t npos = 0;
+    for (size_t i = 0; i < recs.size(); ++i) {
+      for (const RecOp& op : recs[i]->ops)
+        if (op.phase == pl.first && op.level == pl.second) seq[i].push_back(&op);
+      npos = std::max(npos, seq[i].size());
+    }
+    for (size_t pos = 0; pos < npos; ++pos) {
+      std::vector<MultiLaunchFn> fns;  // kernels at this position, first-seen order
+      for (size_t i = 0; i < recs.size(); ++i)
+        if (pos < seq[i].size() && std::find(fns.begin(), fns.end(), seq[i][pos]->fn) == fns.end())
+          fns.push_back(seq[i][pos]->fn);
+      for (MultiLaunchFn fn : fns) {
+        std::vector<std::pair<size_t, const RecOp*>> members;
+        for (size_t i = 0; i < recs.size(); ++i)
+          if (pos < seq[i].size() && seq[i][pos]->fn == fn) members.push_back({i, seq[i][pos]});
+        MergedLaunch m;
+        m.fn = fn;
+        align(64);
+        m.desc_off = blob.size();
+        for (const auto& mb : members) {
+          m.lds = std::max(m.lds, mb.second->lds);
+          blob.insert(blob.end(), mb.second->args.begin(), mb.second->args.end());
+        }
+        align(64);
+        m.tile_off = blob.size();
+        for (size_t j = 0; j < members.size(); ++j) {
+          const unsigned nb = members[j].second->nb;
+          for (unsigned b = 0; b < nb; ++b) {
+            const uint2 t{unsigned(j), b};
+            const auto* pt = reinterpret_cast<const unsigned char*>(&t);
+            blob.insert(blob.end(), pt, pt + sizeof(uint2));
+          }
+          m.nblocks += nb;
+        }
+        if (m.nblocks) out.push_back(m);
+      }
+    }
+  }
+}
+
+__global__ void gather_done_kernel(PcgState* const* __restrict__ st, int n, int* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = st[i]->done;
+}
+
+// Whether the batch can run in lockstep: 2D handles, GMG V(1,1) cycles
+// whose first fine sweep is fused into the PCG update (the launches that
+// have multi-instance forms).
+bool lockstep_eligible(int n, tl_problem* const* hs, int precond) {
+  if (precond != TL_PRECOND_GMG) return false;
+  for (int i = 0; i < n; ++i) {
+    const tl_problem* P = hs[i];
+    if (P->g.dim != 2 || !pcg_fuses_x0(P, precond) || P->mg.gamma > 1 ||
+        P->mg.nlevels > 12)
+      return false;
+  }
+  return true;
+}
+
+int batch_lockstep(int32_t n, tl_problem* const* hs, const double* const* f, double* const* u,
+                   const int32_t* nrhs, const tl_solve_opts* o, int chunk, tl_solve_info* infos) {
+  // systems, their buffer layout in one arena
+  struct Sys { int inst, rhs; SysBufs buf; PcgCtx ctx; Recorder rec[2]; };
+  std::vector<Sys> sys;
+  size_t total = 0;
+  std::vector<std::vector<size_t>> sizes(n);
+  for (int i = 0; i < n; ++i) {
+    sizes[i] = sys_sizes(hs[i]);
+    for (int j = 0; j < nrhs[i]; ++j) {
+      sys.push_back(Sys{i, j, {}, {}, {}});
+      for (size_t b : sizes[i]) total += (b + 255) / 256 * 256;
+    }
+  }
+  DevBuf arena;
+  TL_TRY(arena.alloc(total));
+  struct ArenaGuard { DevBuf& a; ~ArenaGuard() { a.release(); } } guard{arena};
+  {
+    size_t off = 0;
+    for (Sys& sy : sys) {
+      const auto& sz = sizes[sy.inst];
+      auto carve = [&](DevBuf& d, size_t b) {
+        d.ptr = static_cast<unsigned char*>(arena.ptr) + off;
+        d.bytes = b;
+        off += (b + 255) / 256 * 256;
+      };
+      for (int k = 0; k < kSysBufs; ++k) carve(sy.buf.b[k], sz[k]);
+      sy.buf.lv.resize(sz.size() - kSysBufs);
+      for (size_t k = 0; k < sy.buf.lv.size(); ++k) carve(sy.buf.lv[k], sz[kSysBufs + k]);
+    }
+  }
+  hipStream_t s = hs[0]->stream;
+  // initialise every system and record its two iteration parities
+  for (Sys& sy : sys) {
+    tl_problem* P = hs[sy.inst];
+    const int64_t nd = P->ndof;
+    swap_sys(P, sy.buf);
+    const hipStream_t saved = P->stream;
+    P->stream = s;
+    int rc = pcg_context(P, sy.ctx);
+    const double* x0 = o->x0_given ? u[sy.inst] + int64_t(sy.rhs) * nd : nullptr;
+    if (rc == TL_OK) rc = pcg_init(P, sy.ctx, f[sy.inst] + int64_t(sy.rhs) * nd, x0,
+                                   P->xb.as<double>(), o);
+    for (int k = 0; k < 2 && rc == TL_OK; ++k) {
+      P->rec = &sy.rec[k];
+      rc = pcg_iteration(P, sy.ctx, P->xb.as<double>(), o->precond, k);
+      P->rec = nullptr;
+    }
+    P->stream = saved;
+    swap_sys(P, sy.buf);
+    TL_TRY(rc);
+    TL_ARG(!sy.rec[0].unsupported && !sy.rec[1].unsupported,
+           "internal: a lockstep system reached a launch without a multi-instance form");
+  }
+  // state pointers for the convergence checks
+  const int ns = int(sys.size());
+  DevBuf stp, dflags;
+  struct Bufs { DevBuf& a; DevBuf& b; ~Bufs() { a.release(); b.release(); } } bufs{stp, dflags};
+  TL_TRY(stp.alloc(sizeof(PcgState*) * ns));
+  TL_TRY(dflags.alloc(sizeof(int) * ns));
+  std::vector<PcgState*> sts(ns);
+  for (int i = 0; i < ns; ++i) sts[i] = sys[i].ctx.st;
+  TL_HIP(hipMemcpyAsync(stp.ptr, sts.data(), sizeof(PcgState*) * ns, hipMemcpyHostToDevice, s));
+  int* hflags = nullptr;
+  TL_HIP(hipHostMalloc(reinterpret_cast<void**>(&hflags), sizeof(int) * ns, hipHostMallocDefault));
+  struct HostGuard { int* p; ~HostGuard() { (void)hipHostFree(p); } } hguard{hflags};
+  // schedules (rebuilt without the converged systems as they drop out)
+  std::vector<MergedLaunch> sched[2];
+  std::vector<unsigned char> blob[2];
+  std::vector<DevBuf> uploads;  // kept until the end: launches in flight read them
+  struct UpGuard { std::vector<DevBuf>& v; ~UpGuard() { for (auto& d : v) d.release(); } } ug{uploads};
+  std::vector<char> active(ns, 1);
+  const unsigned char* dev_blob[2] = {nullptr, nullptr};
+  auto build = [&]() -> int {
+    for (int par = 0; par < 2; ++par) {
+      std::vector<const Recorder*> recs;
+      for (int i = 0; i < ns; ++i)
+        if (active[i]) recs.push_back(&sys[i].rec[par]);
+      merge_schedule(recs, sched[par], blob[par]);
+      DevBuf d;
+      TL_TRY(d.alloc(std::max<size_t>(blob[par].size(), 64)));
+      uploads.push_back(d);
+      // blob[par] stays unchanged until the stream is synchronised below
+      TL_HIP(hipMemcpyAsync(d.ptr, blob[par].data(), blob[par].size(), hipMemcpyHostToDevice, s));
+      dev_blob[par] = static_cast<const unsigned char*>(d.ptr);
+    }
+    return TL_OK;
+  };
+  TL_TRY(build());
+  int nactive = ns, built_for = ns;
+  int64_t k = 0;
+  for (;;) {
+    for (int c = 0; c < chunk; ++c, ++k) {
+      const int par = int(k & 1);
+      for (const MergedLaunch& m : sched[par])
+        TL_TRY(m.fn(s, dev_blob[par] + m.desc_off,
+                    reinterpret_cast<const uint2*>(dev_blob[par] + m.tile_off), m.nblocks, m.lds));
+    }
+    hipLaunchKernelGGL(gather_done_kernel, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, s,
+                       stp.as<PcgState*>(), ns, dflags.as<int>());
+    TL_CHECK_LAUNCH();
+    TL_HIP(hipMemcpyAsync(hflags, dflags.ptr, sizeof(int) * ns, hipMemcpyDeviceToHost, s));
+    TL_HIP(hipStreamSynchronize(s));
+    nactive = 0;
+    for (int i = 0; i < ns; ++i) {
+      active[i] = !hflags[i];
+      nactive += active[i];
+    }
+    if (nactive == 0 || k > int64_t(o->maxiter) + chunk) break;
+    // drop converged systems from the launches once a quarter of them is done
+    if (nactive <= built_for * 3 / 4) {
+      TL_TRY(build());
+      built_for = nactive;
+    }
+  }
+  // true residuals and results
+  BatchSlot* slots = nullptr;
+  TL_HIP(hipHostMalloc(reinterpret_cast<void**>(&slots), sizeof(BatchSlot) * ns,
+                       hipHostMallocDefault));
+  struct SlotGuard { BatchSlot* p; ~SlotGuard() { (void)hipHostFree(p); } } sguard{slots};
+  for (int i = 0; i < ns; ++i) {
+    Sys& sy = sys[i];
+    tl_problem* P = hs[sy.inst];
+    swap_sys(P, sy.buf);
+    const hipStream_t saved = P->stream;
+    P->stream = s;
+    double* scal = sy.ctx.partials + 3 * kMaxRedBlocks;
+    int rc = pcg_true_residual(P, sy.ctx, P->xb.as<double>(), scal);
+    if (rc == TL_OK &&
+        (hipMemcpyAsync(u[sy.inst] + int64_t(sy.rhs) * P->ndof, P->xb.ptr,
+                        sizeof(double) * P->ndof, hipMemcpyDeviceToDevice, s) != hipSuccess ||
+         hipMemcpyAsync(&slots[i].tr2, scal, sizeof(double), hipMemcpyDeviceToHost, s) !=
+             hipSuccess ||
+         hipMemcpyAsync(&slots[i].hs, sy.ctx.st, sizeof(PcgState), hipMemcpyDeviceToHost, s) !=
+             hipSuccess)) {
+      set_error("lockstep batch: result copies failed");
+      rc = TL_EHIP;
+    }
+    P->stream = saved;
+    swap_sys(P, sy.buf);
+    TL_TRY(rc);
+  }
+  TL_HIP(hipStreamSynchronize(s));
+  bool all_conv = true;
+  std::vector<int64_t> info_at(n, 0);
+  for (int i = 1; i < n; ++i) info_at[i] = info_at[i - 1] + nrhs[i - 1];
+  for (int i = 0; i < ns; ++i) {
+    tl_solve_info* inf = infos ? infos + info_at[sys[i].inst] + sys[i].rhs : nullptr;
+    if (!pcg_info(slots[i].hs, slots[i].tr2, inf)) all_conv = false;
+  }
+  if (!all_conv) {
+    set_error("pcg iteration did not converge for at least one right-hand side");
+    return TL_ENOCONV;
+  }
+  return TL_OK;
+}
 }  // namespace
 
 int tl_batch_solve(int32_t n, tl_problem* const* hs, const double* const* f, double* const* u,
@@ -1763,6 +2200,26 @@ int tl_batch_solve(int32_t n, tl_problem* const* hs, const double* const* f, dou
     TL_ARG(hs[i]->op.have_density, "tl_set_density must be called first");
     TL_ARG(o->precond != TL_PRECOND_GMG || hs[i]->mg.ready, "tl_setup_mg must be called first");
     for (int j = 0; j < i; ++j) TL_ARG(hs[i] != hs[j], "a handle appears twice in the batch");
+  }
+  if (bo && bo->lockstep && lockstep_eligible(n, hs, o->precond)) {
+    // the systems start after the work already queued on each handle's stream
+    for (int i = 1; i < n; ++i)
+      if (hs[i]->stream != hs[0]->stream) {
+        hipEvent_t e = nullptr;
+        TL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        const hipError_t er = hipEventRecord(e, hs[i]->stream);
+        const hipError_t ew = er == hipSuccess ? hipStreamWaitEvent(hs[0]->stream, e, 0) : er;
+        (void)hipEventDestroy(e);
+        TL_HIP(ew);
+      }
+    std::vector<bool> prof(n);
+    for (int i = 0; i < n; ++i) { prof[i] = hs[i]->prof_on; hs[i]->prof_on = false; }
+    const int rc = batch_lockstep(n, hs, f, u, nrhs, o, chunk, infos);
+    for (int i = 0; i < n; ++i) hs[i]->prof_on = prof[i];
+    // later work on the other handles' streams follows the batch
+    for (int i = 1; i < n; ++i)
+      if (hs[i]->stream != hs[0]->stream) (void)hipStreamSynchronize(hs[0]->stream);
+    return rc;
   }
   std::vector<hipStream_t> streams(nstreams, nullptr);
   std::vector<hipEvent_t> ev(n, nullptr);

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <string>
 #include <vector>
 
@@ -104,12 +105,18 @@ __device__ inline void block_sum(double (&v)[NV], double* sh) {
 // device-scope release writes back the XCD's L2, which made the former
 // "last block folds" scheme cost ~50 ns per block.
 template <int NV>
-__device__ inline void store_partials(const double (&part)[NV], double* partials) {
+__device__ inline void store_partials_at(const double (&part)[NV], double* partials,
+                                         unsigned bid) {
   if (threadIdx.x == 0) {
-    const unsigned bid = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
 #pragma unroll
     for (int k = 0; k < NV; ++k) partials[k * kMaxRedBlocks + bid] = part[k];
   }
+}
+
+template <int NV>
+__device__ inline void store_partials(const double (&part)[NV], double* partials) {
+  store_partials_at<NV>(part, partials,
+                        blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z));
 }
 
 // Fold nb partials per value in a fixed order (strided per thread, then the
@@ -212,7 +219,42 @@ struct ProfEvent {
 };
 }  // namespace tl
 
+namespace tl {
+// Launch recording for lockstep batched solves.  While a handle's `rec` is
+// set, the launch helpers of the PCG iteration and the V-cycle do not launch:
+// they append the kernel's multi-instance launcher, its argument struct, its
+// block count and dynamic LDS, tagged with the op's place in the cycle
+// (phase: 0 down the cycle at `level`, 1 up the cycle at `level`, 2 the PCG
+// update after the preconditioner).  tl_batch_solve merges the recordings of
+// many systems and replays each (place, position, launcher) group as one
+// launch over all systems' blocks: multi-instance kernels read their
+// system's argument struct and local block index from a tile table.
+using MultiLaunchFn = int (*)(hipStream_t s, const void* descs, const uint2* tiles,
+                              unsigned nblocks, size_t lds);
+struct RecOp {
+  int phase = 0, level = 0;
+  MultiLaunchFn fn = nullptr;
+  unsigned nb = 0;
+  size_t lds = 0;
+  std::vector<unsigned char> args;
+};
+struct Recorder {
+  std::vector<RecOp> ops;
+  bool unsupported = false;  // a launch site without a multi-instance form was reached
+  template <class A>
+  void add(int phase, int level, MultiLaunchFn fn, const A& a, unsigned nb, size_t lds) {
+    RecOp op;
+    op.phase = phase; op.level = level; op.fn = fn; op.nb = nb; op.lds = lds;
+    op.args.resize(sizeof(A));
+    std::memcpy(op.args.data(), &a, sizeof(A));
+    ops.push_back(std::move(op));
+  }
+};
+}  // namespace tl
+
 struct tl_problem {
+  tl::Recorder* rec = nullptr;  // lockstep batch recording (see Recorder)
+  int rec_phase = 0, rec_level = 0;
   tl::Grid g;
   int ndpn = 1, nd = 8;   // DOFs per node, per element
   int64_t ndof = 0;

@@ -506,15 +506,123 @@ __global__ void prolong_add_kernel(GridD gf, GridD gc, const T* __restrict__ xc,
   prolong_node<NDPN, T, D3>(gf, gc, xc, maskf, xf, n);
 }
 
+// ---------------------------------------------------------------------------
+// Multi-instance forms for lockstep batched solves: one launch covers the
+// blocks of many systems; block b computes local block tiles[b].y of the
+// system whose argument struct is descs[tiles[b].x], with the same per-node
+// arithmetic as the single-system kernel.
+// ---------------------------------------------------------------------------
+template <typename T>
+struct RestrictArgs {
+  GridD gf, gc;
+  const T* rf;
+  const uint8_t* maskc;
+  T* bc;
+  const T* dinvc;
+  T omega;
+  T* xc;
+  const int* done;
+};
+
+template <int NDPN, typename T, bool D3, bool SEED>
+__global__ __launch_bounds__(BS) void restrict_multi_kernel(const RestrictArgs<T>* __restrict__ descs,
+                                                            const uint2* __restrict__ tiles) {
+  const uint2 t = tiles[blockIdx.x];
+  const RestrictArgs<T> a = descs[t.x];
+  if (*a.done) return;
+  const int64_t n = int64_t(t.y) * BS + threadIdx.x;
+  if (n >= a.gc.nnodes) return;
+  restrict_node<NDPN, T, D3, SEED>(a.gf, a.gc, a.rf, a.maskc, a.bc, a.dinvc, a.omega, a.xc, n);
+}
+
+template <int NDPN, typename T, bool D3, bool SEED>
+int restrict_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                          size_t lds) {
+  hipLaunchKernelGGL((restrict_multi_kernel<NDPN, T, D3, SEED>), dim3(nb), dim3(BS), lds, s,
+                     static_cast<const RestrictArgs<T>*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+template <typename T>
+struct ProlongArgs {
+  GridD gf, gc;
+  const T* xc;
+  const uint8_t* maskf;
+  T* xf;
+  const int* done;
+};
+
+template <int NDPN, typename T, bool D3>
+__global__ __launch_bounds__(BS) void prolong_multi_kernel(const ProlongArgs<T>* __restrict__ descs,
+                                                           const uint2* __restrict__ tiles) {
+  const uint2 t = tiles[blockIdx.x];
+  const ProlongArgs<T> a = descs[t.x];
+  if (*a.done) return;
+  const int64_t n = int64_t(t.y) * BS + threadIdx.x;
+  if (n >= a.gf.nnodes) return;
+  prolong_node<NDPN, T, D3>(a.gf, a.gc, a.xc, a.maskf, a.xf, n);
+}
+
+template <int NDPN, typename T, bool D3>
+int prolong_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                         size_t lds) {
+  hipLaunchKernelGGL((prolong_multi_kernel<NDPN, T, D3>), dim3(nb), dim3(BS), lds, s,
+                     static_cast<const ProlongArgs<T>*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+template <typename T>
+struct StArgs {
+  GridD g;
+  const T* sth;
+  const T* x;
+  const T* b;
+  const T* dinv;
+  T omega;
+  const uint8_t* mask;
+  T* y;
+  const int* done;
+  CoarseCorr<T> cc;
+};
+
+template <int NDPN, typename T, int MODE, int DIMU, bool PRO>
+__global__ __launch_bounds__(BS) void st_apply_multi_kernel(const StArgs<T>* __restrict__ descs,
+                                                            const uint2* __restrict__ tiles) {
+  const uint2 t = tiles[blockIdx.x];
+  const StArgs<T> a = descs[t.x];
+  if (*a.done) return;
+  const int64_t n = int64_t(t.y) * BS + threadIdx.x;
+  if (n >= a.g.nnodes) return;
+  st_apply_node<NDPN, T, MODE, DIMU, PRO>(a.g, a.sth, a.x, a.b, a.dinv, a.omega, a.mask, a.y, n,
+                                          a.cc);
+}
+
+template <int NDPN, typename T, int MODE, int DIMU, bool PRO>
+int st_apply_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                          size_t lds) {
+  hipLaunchKernelGGL((st_apply_multi_kernel<NDPN, T, MODE, DIMU, PRO>), dim3(nb), dim3(BS), lds,
+                     s, static_cast<const StArgs<T>*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
 template <int NDPN, typename T>
 int launch_restrict(tl_problem* P, const Grid& gf, const Grid& gc, const T* rf,
                     const uint8_t* maskc, T* bc, const T* dinvc, T omega, T* xc,
                     const int* done) {
   const dim3 grid(unsigned((gc.nnodes + BS - 1) / BS));
   auto go = [&](auto d3_c, auto seed_c) {
-    hipLaunchKernelGGL((restrict_kernel<NDPN, T, decltype(d3_c)::value, decltype(seed_c)::value>),
-                       grid, dim3(BS), 0, P->stream, to_d(gf), to_d(gc), rf, maskc, bc, dinvc,
-                       omega, xc, done);
+    constexpr bool D3_ = decltype(d3_c)::value, SEED_ = decltype(seed_c)::value;
+    if (P->rec) {
+      P->rec->add(P->rec_phase, P->rec_level, &restrict_multi_launch<NDPN, T, D3_, SEED_>,
+                  RestrictArgs<T>{to_d(gf), to_d(gc), rf, maskc, bc, dinvc, omega, xc, done},
+                  grid.x, 0);
+      return;
+    }
+    hipLaunchKernelGGL((restrict_kernel<NDPN, T, D3_, SEED_>), grid, dim3(BS), 0, P->stream,
+                       to_d(gf), to_d(gc), rf, maskc, bc, dinvc, omega, xc, done);
   };
   using F = std::false_type;
   using Tr = std::true_type;
@@ -528,6 +636,14 @@ template <int NDPN, typename T>
 int launch_prolong(tl_problem* P, const Grid& gf, const Grid& gc, const T* xc,
                    const uint8_t* maskf, T* xf, const int* done) {
   const dim3 grid(unsigned((gf.nnodes + BS - 1) / BS));
+  if (P->rec) {
+    const ProlongArgs<T> a{to_d(gf), to_d(gc), xc, maskf, xf, done};
+    if (gf.dim == 3)
+      P->rec->add(P->rec_phase, P->rec_level, &prolong_multi_launch<NDPN, T, true>, a, grid.x, 0);
+    else
+      P->rec->add(P->rec_phase, P->rec_level, &prolong_multi_launch<NDPN, T, false>, a, grid.x, 0);
+    return TL_OK;
+  }
   if (gf.dim == 3)
     hipLaunchKernelGGL((prolong_add_kernel<NDPN, T, true>), grid, dim3(BS), 0, P->stream, to_d(gf),
                        to_d(gc), xc, maskf, xf, done);
@@ -576,6 +692,36 @@ __global__ __launch_bounds__(BS) void dense_gemv_kernel(const double* __restrict
   if (row >= n) return;
   gemv_row<T>(A, np, n, b, x, row, threadIdx.x & 63);
 }
+
+template <typename T>
+struct GemvArgs {
+  const double* A;
+  int64_t np, n;
+  const T* b;
+  T* x;
+  const int* done;
+};
+
+template <typename T>
+__global__ __launch_bounds__(BS) void gemv_multi_kernel(const GemvArgs<T>* __restrict__ descs,
+                                                        const uint2* __restrict__ tiles) {
+  const uint2 t = tiles[blockIdx.x];
+  const GemvArgs<T> a = descs[t.x];
+  if (*a.done) return;
+  const int64_t row = int64_t(t.y) * (BS / 64) + (threadIdx.x >> 6);
+  if (row >= a.n) return;
+  gemv_row<T>(a.A, a.np, a.n, a.b, a.x, row, threadIdx.x & 63);
+}
+
+template <typename T>
+int gemv_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                      size_t lds) {
+  hipLaunchKernelGGL((gemv_multi_kernel<T>), dim3(nb), dim3(BS), lds, s,
+                     static_cast<const GemvArgs<T>*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
 
 // ---------------------------------------------------------------------------
 // dense coarsest operator and its in-place blocked Gauss-Jordan inverse
@@ -847,6 +993,11 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
   const uint8_t* mk = L.mask.as<uint8_t>();
   auto go = [&](auto mode_c, auto dim_c) {
     constexpr int MODE_ = decltype(mode_c)::value, DIM_ = decltype(dim_c)::value;
+    if (P->rec) {
+      P->rec->add(P->rec_phase, P->rec_level, &st_apply_multi_launch<NDPN, T, MODE_, DIM_, false>,
+                  StArgs<T>{gd, st, x, b, dinv, om, mk, y, done, CoarseCorr<T>{}}, nb, 0);
+      return;
+    }
     hipLaunchKernelGGL((st_apply_kernel<NDPN, T, MODE_, DIM_>), dim3(nb), dim3(BS), 0, P->stream,
                        gd, st, x, b, dinv, om, mk, y, done, CoarseCorr<T>{});
   };
@@ -903,9 +1054,15 @@ int st_smooth_corrected(tl_problem* P, int l, const T* x, const T* xc, const T* 
   const GridD gd = to_d(L.g);
   const T om = T(P->mg.omega);
   (void)dimu;  // 2: fuse_correction levels are 2D
-  if constexpr (NDPN != 3)
+  if constexpr (NDPN != 3) {
+    if (P->rec) {
+      P->rec->add(P->rec_phase, P->rec_level, &st_apply_multi_launch<NDPN, T, 1, 2, true>,
+                  StArgs<T>{gd, st, x, b, dinv, om, cc.mask, y, done, cc}, nb, 0);
+      return prof_end(P, ev);
+    }
     hipLaunchKernelGGL((st_apply_kernel<NDPN, T, 1, 2, true>), dim3(nb), dim3(BS), 0, P->stream,
                        gd, st, x, b, dinv, om, cc.mask, y, done, cc);
+  }
   TL_CHECK_LAUNCH();
   return prof_end(P, ev);
 }
@@ -920,9 +1077,17 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
   const unsigned nb = unsigned((n + BS - 1) / BS);
   TL_NONNULL(L.vb.ptr); TL_NONNULL(L.vx.ptr); TL_NONNULL(L.vt.ptr); TL_NONNULL(L.vr.ptr);
   TL_NONNULL(L.mask.ptr);
+  P->rec_phase = 0;  // down the cycle at level l
+  P->rec_level = l;
   if (l == M.nlevels - 1) {
     TL_NONNULL(M.cinv.ptr);
     TL_ARG(M.cn == n, "internal: coarsest size mismatch");
+    if (P->rec) {
+      P->rec->add(0, l, &gemv_multi_launch<T>,
+                  GemvArgs<T>{M.cinv.as<double>(), M.cnp, n, vp<T>(L.vb), vp<T>(L.vx), done},
+                  unsigned((n + 3) / 4), 0);
+      return TL_OK;
+    }
     hipLaunchKernelGGL(dense_gemv_kernel<T>, dim3(unsigned((n + 3) / 4)), dim3(BS), 0, P->stream,
                        M.cinv.as<double>(), M.cnp, n, vp<T>(L.vb), vp<T>(L.vx), done);
     TL_CHECK_LAUNCH();
@@ -951,6 +1116,8 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
     Level& C = M.lv[l + 1];
     TL_TRY((restrict_to<NDPN, T>(P, l, L.g, vp<T>(L.vr), done)));
     TL_TRY((vcycle_level<NDPN, T>(P, l + 1, done)));
+    P->rec_phase = 1;  // up the cycle at level l
+    P->rec_level = l;
     // coarse correction and post-smoothing; on the fuse_correction levels the
     // correction is added inside the first sweep
     int s0 = 0;
@@ -990,7 +1157,13 @@ int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done, bool
   const void* dinv = std::is_same<T, double>::value ? P->op.dinv64.ptr : P->op.dinv32.ptr;
   T* x = vp<T>(L.vx);
   T* t = vp<T>(L.vt);
+  P->rec_phase = 0;
+  P->rec_level = 0;
   if (!x0_ready) {
+    if (P->rec) {  // no multi-instance form: the lockstep batch needs the fused first sweep
+      P->rec->unsupported = true;
+      return TL_OK;
+    }
     hipLaunchKernelGGL((jacobi_zero_kernel<T, double>), dim3(gsz(n)), dim3(BS), 0, P->stream, r,
                        static_cast<const T*>(dinv), T(M.omega), x, n, done);
     TL_CHECK_LAUNCH();
@@ -1005,6 +1178,8 @@ int vcycle_fine(tl_problem* P, const double* r, double* z, const int* done, bool
                        nullptr, nullptr, 0, done, false));
   TL_TRY((restrict_to<NDPN, T>(P, 0, L.g, vp<T>(L.vr), done)));
   TL_TRY((vcycle_level<NDPN, T>(P, 1, done)));
+  P->rec_phase = 1;
+  P->rec_level = 0;
   // the fine grid's Dirichlet mask is the operator's (level 0 has no copy)
   TL_TRY((launch_prolong<NDPN, T>(P, L.g, C.g, vp<T>(C.vx), P->op.mask.as<uint8_t>(), x, done)));
   for (int s = 0; s + 1 < M.nu2; ++s) {
