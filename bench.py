@@ -228,7 +228,7 @@ def kernel_kinds(dims, vcycle, nlevels, fp64_mfma=False):
     }
 
 
-def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True):
+def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True, lockstep_only=False):
     """Batched uneven 2D solves: 32 Q4 elasticity grids (4 load cases each)
     and 8 Q4 heat grids from workloads.batched_instances(seed=0), FP64 outer
     PCG with FP32 V-cycles to relres 1e-8.  A step = set_density +
@@ -273,20 +273,24 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True):
     ms_step = timed(lambda: (setup_all(), solve_all()), steps)
     ms_setup = timed(setup_all, steps)
     ms_solve = timed(solve_all, steps)  # lockstep: all systems in every launch
-    solve_all(lockstep=False)  # captures every instance's chunk graph for this hierarchy
-    ms_streams = timed(lambda: solve_all(lockstep=False), steps)
-    ms_eager = timed(lambda: solve_all(graphs=False, lockstep=False), steps)
-    ms_one_stream = timed(lambda: solve_all(streams=1, lockstep=False), steps)
-    ms_seq = timed(sequential, 1)
+    ms_streams = ms_eager = ms_one_stream = ms_seq = None
+    if not lockstep_only:  # the other ways to run the same systems
+        solve_all(lockstep=False)  # captures every instance's chunk graph for this hierarchy
+        ms_streams = timed(lambda: solve_all(lockstep=False), steps)
+        ms_eager = timed(lambda: solve_all(graphs=False, lockstep=False), steps)
+        ms_one_stream = timed(lambda: solve_all(streams=1, lockstep=False), steps)
+        ms_seq = timed(sequential, 1)
     its = [i["iterations"] for lst in infos for i in lst]
     rr = [i["relres_true"] for lst in infos for i in lst]
     dof_rhs = sum(op.ndof * f.shape[0] for op, f in zip(ops, fs))
     hist = {}
     for t in tiles:
         hist[str(t["fp64"])] = hist.get(str(t["fp64"]), 0) + 1
+    fmt = lambda v: "-" if v is None else f"{v:.1f}"
+    r2 = lambda v: None if v is None else round(v, 2)
     log(f"batched: step {ms_step:.1f} ms (setup {ms_setup:.1f}, lockstep batch solve "
-        f"{ms_solve:.1f}, per-instance graphs on {nstreams} streams {ms_streams:.1f}, eager "
-        f"{ms_eager:.1f}, one stream {ms_one_stream:.1f}, one at a time {ms_seq:.1f}); "
+        f"{ms_solve:.1f}, per-instance graphs on {nstreams} streams {fmt(ms_streams)}, eager "
+        f"{fmt(ms_eager)}, one stream {fmt(ms_one_stream)}, one at a time {fmt(ms_seq)}); "
         f"iterations {min(its)}..{max(its)}, max true relres {max(rr):.2e}")
     out = {
         "workload": "32 Q4 elasticity grids (nelx 128..1024, nely 64..512, multiples of 32, "
@@ -298,11 +302,11 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True):
         "dof_rhs_total": int(dof_rhs),
         "ms_per_step": round(ms_step, 2), "ms_setup": round(ms_setup, 2),
         "ms_batch_solve": round(ms_solve, 2), "batch_solve_mode": "lockstep",
-        "ms_batch_solve_stream_graphs": round(ms_streams, 2),
-        "ms_batch_solve_eager": round(ms_eager, 2),
-        "ms_batch_solve_one_stream": round(ms_one_stream, 2),
-        "ms_one_at_a_time": round(ms_seq, 2),
-        "speedup_batch_vs_one_at_a_time": round(ms_seq / ms_solve, 2),
+        "ms_batch_solve_stream_graphs": r2(ms_streams),
+        "ms_batch_solve_eager": r2(ms_eager),
+        "ms_batch_solve_one_stream": r2(ms_one_stream),
+        "ms_one_at_a_time": r2(ms_seq),
+        "speedup_batch_vs_one_at_a_time": None if ms_seq is None else round(ms_seq / ms_solve, 2),
         "nstreams": nstreams, "chunk": chunk,
         "pcg_iterations": {"min": min(its), "max": max(its), "mean": round(float(np.mean(its)), 1)},
         "relres_true_max": max(rr),
@@ -452,6 +456,8 @@ def main():
                     help="run only the batched uneven-grid workload and print its JSON line")
     ap.add_argument("--no-batched", action="store_true",
                     help="skip the batched uneven-grid workload in the full run")
+    ap.add_argument("--batched-lockstep-only", action="store_true",
+                    help="with --batched: time only the lockstep batch solve")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (5 launches) so that "
                          "FETCH_SIZE/WRITE_SIZE counters can be scaled to bytes")
@@ -475,6 +481,7 @@ def main():
     from topoptlab_amd import _lib
     if args.batched:
         res = bench_batched(torch.device("cuda", 0), steps=max(1, args.steps - 1),
+                            lockstep_only=args.batched_lockstep_only,
                             cpu_sample=not args.no_cpu_baseline)
         print(json.dumps({"metric": "ms per batched step (40 uneven 2D instances, 136 systems)",
                           "value": res["ms_per_step"], "unit": "ms", "n_gpus": 1,

@@ -128,6 +128,7 @@ struct KuArgs {
   int fin;
   const int* done;
   int nelx, nely, nelz, nx, ny, nz, plane, ncol;  // ncol: 3D tile columns
+  int ty;  // 2D workgroup shape the launch uses (0: node line), for the multi-instance form
   // group values of a pattern-matched element matrix (see ke_pattern.hpp)
   double kv64[8];
   float kv32[8];
@@ -583,11 +584,10 @@ __global__ __launch_bounds__(BS) void ku3d_kernel(KuArgs a) {
   }
 }
 
+// lds: (BS + 2 ny + 2) * NDPN values of T
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
-__device__ __forceinline__ void ku2d_body(const KuArgs& a, unsigned bid) {
+__device__ __forceinline__ void ku2d_body(const KuArgs& a, unsigned bid, T* lds) {
   if (a.done && *a.done) return;
-  extern __shared__ double lds_raw[];
-  T* lds = reinterpret_cast<T*>(lds_raw);
   const int64_t j0 = int64_t(bid) * BS;
   const int64_t j = j0 + threadIdx.x;
   const bool active = j < a.plane;
@@ -630,7 +630,8 @@ __device__ __forceinline__ void ku2d_body(const KuArgs& a, unsigned bid) {
 
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
 __global__ __launch_bounds__(BS) void ku2d_kernel(KuArgs a) {
-  ku2d_body<NDPN, T, MODE, TB, TO, FUSEP, FAST>(a, blockIdx.x);
+  extern __shared__ double lds_raw[];
+  ku2d_body<NDPN, T, MODE, TB, TO, FUSEP, FAST>(a, blockIdx.x, reinterpret_cast<T*>(lds_raw));
 }
 
 static int launch_fold(tl_problem* P, unsigned nb, int fin, double* out, const int* done);
@@ -641,13 +642,16 @@ static int launch_fold(tl_problem* P, unsigned nb, int fin, double* out, const i
 // nodes for BS computed, a halo that grows with the grid height; the tile's
 // does not, but its rows are shorter.  tl_autotune picks the faster shape
 // per grid.
+template <int TY>
+constexpr int ku2t_halo_nodes() { return (BS / TY + 2) * (TY + 2); }
+
+// lds: ku2t_halo_nodes<TY>() * NDPN values of T
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
-__device__ __forceinline__ void ku2t_body(const KuArgs& a, unsigned bid) {
+__device__ __forceinline__ void ku2t_body(const KuArgs& a, unsigned bid, T* lds) {
   if (a.done && *a.done) return;
   constexpr int TX = BS / TY;
   constexpr int HY = TY + 2, HN = (TX + 2) * HY;
   static_assert(TX * TY == BS, "tile must hold BS nodes");
-  __shared__ T lds[HN * NDPN];
   const int ntx = (a.nx + TX - 1) / TX;
   const int x0 = int(bid % unsigned(ntx)) * TX, y0 = int(bid / unsigned(ntx)) * TY;
   const int lx = threadIdx.x / TY, ly = threadIdx.x % TY;
@@ -705,38 +709,37 @@ __device__ __forceinline__ void ku2t_body(const KuArgs& a, unsigned bid) {
 
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
 __global__ __launch_bounds__(BS) void ku2t_kernel(KuArgs a) {
-  ku2t_body<NDPN, T, MODE, TB, TO, FUSEP, TY>(a, blockIdx.x);
+  __shared__ T lds[ku2t_halo_nodes<TY>() * NDPN];
+  ku2t_body<NDPN, T, MODE, TB, TO, FUSEP, TY>(a, blockIdx.x, lds);
 }
 
-// Multi-instance forms of the 2D operator for lockstep batched solves (see
-// tl::Recorder): block b runs local block tiles[b].y of system tiles[b].x.
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
-__global__ __launch_bounds__(BS) void ku2t_multi_kernel(const KuArgs* __restrict__ descs,
-                                                        const uint2* __restrict__ tiles) {
+// Multi-instance form of the 2D operator for lockstep batched solves (see
+// tl::Recorder): block b runs local block tiles[b].y of system tiles[b].x in
+// that system's workgroup shape (node line or 16/32/64-high tile), so one
+// launch serves systems whatever shape their autotuning chose.  All shapes
+// stage into one dynamic LDS buffer, sized for the largest among the systems.
+template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
+__global__ __launch_bounds__(BS) void ku2_multi_kernel(const KuArgs* __restrict__ descs,
+                                                       const uint2* __restrict__ tiles) {
+  extern __shared__ double lds_raw[];
+  T* lds = reinterpret_cast<T*>(lds_raw);
   const uint2 t = tiles[blockIdx.x];
-  ku2t_body<NDPN, T, MODE, TB, TO, FUSEP, TY>(descs[t.x], t.y);
-}
-
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, int TY>
-static int ku2t_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
-                             size_t lds) {
-  hipLaunchKernelGGL((ku2t_multi_kernel<NDPN, T, MODE, TB, TO, FUSEP, TY>), dim3(nb), dim3(BS),
-                     lds, s, static_cast<const KuArgs*>(descs), tiles);
-  TL_CHECK_LAUNCH();
-  return TL_OK;
+  const KuArgs& a = descs[t.x];
+  if constexpr (NDPN <= 2 && FAST) {
+    switch (a.ty) {
+      case 16: ku2t_body<NDPN, T, MODE, TB, TO, FUSEP, 16>(a, t.y, lds); return;
+      case 32: ku2t_body<NDPN, T, MODE, TB, TO, FUSEP, 32>(a, t.y, lds); return;
+      case 64: ku2t_body<NDPN, T, MODE, TB, TO, FUSEP, 64>(a, t.y, lds); return;
+      default: break;
+    }
+  }
+  ku2d_body<NDPN, T, MODE, TB, TO, FUSEP, FAST>(a, t.y, lds);
 }
 
 template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
-__global__ __launch_bounds__(BS) void ku2d_multi_kernel(const KuArgs* __restrict__ descs,
-                                                        const uint2* __restrict__ tiles) {
-  const uint2 t = tiles[blockIdx.x];
-  ku2d_body<NDPN, T, MODE, TB, TO, FUSEP, FAST>(descs[t.x], t.y);
-}
-
-template <int NDPN, typename T, int MODE, typename TB, typename TO, bool FUSEP, bool FAST>
-static int ku2d_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
-                             size_t lds) {
-  hipLaunchKernelGGL((ku2d_multi_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), dim3(nb), dim3(BS),
+static int ku2_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                            size_t lds) {
+  hipLaunchKernelGGL((ku2_multi_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), dim3(nb), dim3(BS),
                      lds, s, static_cast<const KuArgs*>(descs), tiles);
   TL_CHECK_LAUNCH();
   return TL_OK;
@@ -777,8 +780,10 @@ static int launch_ku2t(const tl_problem* P, const KuArgs& a) {
     return TL_EINVAL;
   }
   if (P->rec) {
-    P->rec->add(P->rec_phase, P->rec_level, &ku2t_multi_launch<NDPN, T, MODE, TB, TO, FUSEP, TY>,
-                a, unsigned(nb), 0);
+    KuArgs ar = a;
+    ar.ty = TY;
+    P->rec->add(P->rec_phase, P->rec_level, &ku2_multi_launch<NDPN, T, MODE, TB, TO, FUSEP, true>,
+                ar, unsigned(nb), size_t(ku2t_halo_nodes<TY>()) * NDPN * sizeof(T));
   } else {
     hipLaunchKernelGGL((ku2t_kernel<NDPN, T, MODE, TB, TO, FUSEP, TY>), dim3(unsigned(nb)),
                        dim3(BS), 0, P->stream, a);
@@ -850,8 +855,10 @@ static int launch_ku_impl(const tl_problem* P, KuArgs a) {
       return TL_EINVAL;
     }
     if (P->rec) {
+      KuArgs ar = a;
+      ar.ty = 0;
       P->rec->add(P->rec_phase, P->rec_level,
-                  &ku2d_multi_launch<NDPN, T, MODE, TB, TO, FUSEP, FAST>, a, nbx,
+                  &ku2_multi_launch<NDPN, T, MODE, TB, TO, FUSEP, FAST>, ar, nbx,
                   size_t(L) * NDPN * sizeof(T));
     } else {
       hipLaunchKernelGGL((ku2d_kernel<NDPN, T, MODE, TB, TO, FUSEP, FAST>), dim3(nbx), dim3(BS),
@@ -1985,20 +1992,40 @@ void merge_schedule(const std::vector<const Recorder*>& recs, std::vector<Merged
         m.fn = fn;
         align(64);
         m.desc_off = blob.size();
-        for (const auto& mb : members) {
-          m.lds = std::max(m.lds, mb.second->lds);
-          blob.insert(blob.end(), mb.second->args.begin(), mb.second->args.end());
+        std::vector<unsigned> nbs;  // blocks per argument struct
+        const RecOp* first = members[0].second;
+        if (first->pack) {
+          // systems with the same share key in packs of up to kRecPack
+          m.fn = first->pack_fn;
+          std::vector<const void*> keys;
+          for (const auto& mb : members)
+            if (std::find(keys.begin(), keys.end(), mb.second->share) == keys.end())
+              keys.push_back(mb.second->share);
+          for (const void* key : keys) {
+            std::vector<const RecOp*> grp;
+            for (const auto& mb : members)
+              if (mb.second->share == key) grp.push_back(mb.second);
+            for (size_t g0 = 0; g0 < grp.size(); g0 += kRecPack) {
+              const int cnt = int(std::min<size_t>(kRecPack, grp.size() - g0));
+              nbs.push_back(first->pack(grp.data() + g0, cnt, blob));
+            }
+          }
+        } else {
+          for (const auto& mb : members) {
+            m.lds = std::max(m.lds, mb.second->lds);
+            blob.insert(blob.end(), mb.second->args.begin(), mb.second->args.end());
+            nbs.push_back(mb.second->nb);
+          }
         }
         align(64);
         m.tile_off = blob.size();
-        for (size_t j = 0; j < members.size(); ++j) {
-          const unsigned nb = members[j].second->nb;
-          for (unsigned b = 0; b < nb; ++b) {
+        for (size_t j = 0; j < nbs.size(); ++j) {
+          for (unsigned b = 0; b < nbs[j]; ++b) {
             const uint2 t{unsigned(j), b};
             const auto* pt = reinterpret_cast<const unsigned char*>(&t);
             blob.insert(blob.end(), pt, pt + sizeof(uint2));
           }
-          m.nblocks += nb;
+          m.nblocks += nbs[j];
         }
         if (m.nblocks) out.push_back(m);
       }

@@ -231,12 +231,20 @@ namespace tl {
 // system's argument struct and local block index from a tile table.
 using MultiLaunchFn = int (*)(hipStream_t s, const void* descs, const uint2* tiles,
                               unsigned nblocks, size_t lds);
+struct RecOp;
+// Packs up to kRecPack ops with the same `share` key (e.g. systems reading
+// one matrix) into one argument struct appended to out; returns its blocks.
+constexpr int kRecPack = 4;
+using PackFn = unsigned (*)(const RecOp* const* ops, int nops, std::vector<unsigned char>& out);
 struct RecOp {
   int phase = 0, level = 0;
   MultiLaunchFn fn = nullptr;
   unsigned nb = 0;
   size_t lds = 0;
   std::vector<unsigned char> args;
+  const void* share = nullptr;     // ops with the same key may be packed
+  PackFn pack = nullptr;           // how (nullptr: never packed)
+  MultiLaunchFn pack_fn = nullptr;  // launcher of the packed argument structs
 };
 struct Recorder {
   std::vector<RecOp> ops;

@@ -654,6 +654,47 @@ int launch_prolong(tl_problem* P, const Grid& gf, const Grid& gc, const T* xc,
   return TL_OK;
 }
 
+// x_v[row] = (Cinv b_v)[row] for NV vectors sharing one pass over the row
+// (one wavefront per row; inactive vectors skipped); each vector's sum takes
+// the terms in the same order as gemv_row below.
+template <typename T, int NV>
+__device__ __forceinline__ void gemv_row_nv(const double* __restrict__ A, int64_t np, int64_t n,
+                                            const T* const (&b)[NV], T* const (&x)[NV],
+                                            const bool (&on)[NV], int64_t row, int lane) {
+  const double* __restrict__ Ar = A + row * np;
+  double s[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) s[v] = 0.0;
+  constexpr int U = 8;
+  int64_t j = lane;
+  for (; j + 64 * (U - 1) < n; j += 64 * U) {
+    double a[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) a[u] = Ar[j + 64 * u];
+#pragma unroll
+    for (int v = 0; v < NV; ++v) {
+      if (!on[v]) continue;
+      double w[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) w[u] = double(b[v][j + 64 * u]);
+#pragma unroll
+      for (int u = 0; u < U; ++u) s[v] += a[u] * w[u];
+    }
+  }
+  for (; j < n; j += 64) {
+    const double a = Ar[j];
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      if (on[v]) s[v] += a * double(b[v][j]);
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    if (!on[v]) continue;
+    const double t = wave_sum(s[v]);
+    if (lane == 0) x[v][row] = T(t);
+  }
+}
+
 // x[row] = (Cinv b)[row], one wavefront per row (Cinv fp64, padded leading
 // dimension np)
 template <typename T>
@@ -720,6 +761,67 @@ int gemv_multi_launch(hipStream_t s, const void* descs, const uint2* tiles, unsi
                      static_cast<const GemvArgs<T>*>(descs), tiles);
   TL_CHECK_LAUNCH();
   return TL_OK;
+}
+
+// Up to kRecPack systems multiplying one coarsest inverse (the load cases of
+// one instance): the matrix is read once for all of them.
+template <typename T>
+struct GemvPackArgs {
+  const double* A;
+  int64_t np, n;
+  int nv;
+  const T* b[kRecPack];
+  T* x[kRecPack];
+  const int* done[kRecPack];
+};
+
+template <typename T>
+__global__ __launch_bounds__(BS) void gemv_pack_kernel(const GemvPackArgs<T>* __restrict__ descs,
+                                                       const uint2* __restrict__ tiles) {
+  const uint2 t = tiles[blockIdx.x];
+  const GemvPackArgs<T>& a = descs[t.x];
+  bool on[kRecPack];
+  bool any = false;
+#pragma unroll
+  for (int v = 0; v < kRecPack; ++v) {
+    on[v] = v < a.nv && !*a.done[v];
+    any = any || on[v];
+  }
+  if (!any) return;
+  const int64_t row = int64_t(t.y) * (BS / 64) + (threadIdx.x >> 6);
+  if (row >= a.n) return;
+  const T* b[kRecPack];
+  T* x[kRecPack];
+#pragma unroll
+  for (int v = 0; v < kRecPack; ++v) { b[v] = a.b[v]; x[v] = a.x[v]; }
+  gemv_row_nv<T, kRecPack>(a.A, a.np, a.n, b, x, on, row, threadIdx.x & 63);
+}
+
+template <typename T>
+int gemv_pack_launch(hipStream_t s, const void* descs, const uint2* tiles, unsigned nb,
+                     size_t lds) {
+  hipLaunchKernelGGL((gemv_pack_kernel<T>), dim3(nb), dim3(BS), lds, s,
+                     static_cast<const GemvPackArgs<T>*>(descs), tiles);
+  TL_CHECK_LAUNCH();
+  return TL_OK;
+}
+
+template <typename T>
+unsigned gemv_pack(const RecOp* const* ops, int nops, std::vector<unsigned char>& out) {
+  GemvPackArgs<T> p{};
+  for (int k = 0; k < nops; ++k) {
+    GemvArgs<T> g;
+    std::memcpy(&g, ops[k]->args.data(), sizeof(g));
+    if (k == 0) { p.A = g.A; p.np = g.np; p.n = g.n; }
+    p.b[k] = g.b;
+    p.x[k] = g.x;
+    p.done[k] = g.done;
+  }
+  p.nv = nops;
+  for (int k = nops; k < kRecPack; ++k) p.done[k] = p.done[0];
+  const auto* c = reinterpret_cast<const unsigned char*>(&p);
+  out.insert(out.end(), c, c + sizeof(p));
+  return unsigned((p.n + 3) / 4);
 }
 
 
@@ -1086,6 +1188,10 @@ int vcycle_level(tl_problem* P, int l, const int* done) {
       P->rec->add(0, l, &gemv_multi_launch<T>,
                   GemvArgs<T>{M.cinv.as<double>(), M.cnp, n, vp<T>(L.vb), vp<T>(L.vx), done},
                   unsigned((n + 3) / 4), 0);
+      RecOp& op = P->rec->ops.back();  // systems of one instance share the inverse
+      op.share = M.cinv.ptr;
+      op.pack = &gemv_pack<T>;
+      op.pack_fn = &gemv_pack_launch<T>;
       return TL_OK;
     }
     hipLaunchKernelGGL(dense_gemv_kernel<T>, dim3(unsigned((n + 3) / 4)), dim3(BS), 0, P->stream,
