@@ -421,6 +421,60 @@ def bench_mma(dev, n=1_769_472, reps=3):
             "rel_err_vs_oracle": err, "cpu_oracle_ms": round(1e3 * tc, 1), "cpu_cores": 1}
 
 
+def bench_learned_oc(dev, net, dims, levels, steps):
+    """The learned initial guess on a design of its training distribution at
+    the benchmark size: 8 OC iterations of a cantilever with a seeded point
+    load (the network was trained on such runs at half the size).  Plain
+    zero-guess GMG-PCG against PCG from the learned guess, FP64 V-cycle and
+    FP16 MFMA W-cycle, relres 1e-8; iterations, ms per solve (network
+    inference and energy scaling included) and how often the guess was
+    rejected."""
+    import torch
+    from topoptlab_amd import learned
+    from topoptlab_amd.elements import lk_linear_elast_3d
+    from topoptlab_amd.fem import StiffnessOperator
+    xPhys, f, fixed = learned.oc_design(*dims, seed=7, iters=8, device=dev)
+    op = StiffnessOperator(*dims, KE=lk_linear_elast_3d(), fixed=fixed, device=dev)
+    out = {"design": f"{dims[0]}x{dims[1]}x{dims[2]} cantilever, seeded point load (seed 7), "
+                     f"8 OC iterations (vf 0.3, rmin 1.5)"}
+    for vc, g in (("fp64", 1), ("fp16", 2)):
+        op.set_density(xPhys)
+        op.setup_mg(nlevels=levels, precision=vc, gamma=g, gamma_from=2)
+        res = {}
+
+        def plain():
+            u, inf = op.solve(f, rtol=1e-8, check_every=16)
+            return u, dict(inf, learned=False)
+
+        def with_net():
+            return learned.solve_with_learned_guess(net, op, xPhys, f, rtol=1e-8,
+                                                    check_every=16)
+
+        for name, use in (("plain", False), ("learned", True)):
+            run = with_net if use else plain
+            run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fb = 0
+            for _ in range(steps):
+                _, inf = run()
+                fb += 0 if inf.get("learned") else 1
+            torch.cuda.synchronize()
+            res[name] = {"ms": round(1e3 * (time.perf_counter() - t0) / steps, 3),
+                         "pcg_iterations": inf["iterations"], "relres_true": inf["relres_true"]}
+            if use:
+                res[name]["fallbacks"] = f"{fb} of {steps}"
+                res[name]["alpha"] = inf.get("alpha")
+        key = "fp64_vcycle" if vc == "fp64" else "fp16_mfma_wcycle_l2"
+        out[key] = res
+        log(f"learned on OC design, {key}: plain {res['plain']['pcg_iterations']} its "
+            f"{res['plain']['ms']:.2f} ms, learned {res['learned']['pcg_iterations']} its "
+            f"{res['learned']['ms']:.2f} ms (alpha {res['learned']['alpha']:.3f}, fallbacks "
+            f"{res['learned']['fallbacks']})")
+    op.close()
+    return out
+
+
 def roofline_entry(kind_def, avg_ms, traffic):
     name, bytes_alg, flops, bound, peak = kind_def
     if bound == "mfma":
@@ -615,6 +669,9 @@ def main():
         base = variants["fp64_vcycle"]["ms"]
         for v in variants.values():
             v["speedup_vs_fp64_vcycle"] = round(base / v["ms"], 3)
+        if net is not None:
+            variants["learned_on_oc_design"] = bench_learned_oc(dev, net, dims, args.levels,
+                                                                args.steps)
         variants["reference"] = {"vcycle": "fp64", "rtol": 1e-12,
                                  "pcg_iterations": inf_ref["iterations"], "compliance": c_ref}
 

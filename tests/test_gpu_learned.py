@@ -180,3 +180,31 @@ def test_training_reduces_loss_and_learned_guess_gains(gpu):
         "energy captured", [f"{c:.3f}" for c in captured])
     assert all(a <= b for a, b in zip(its1, its0))
     assert min(captured) > 0.02 and np.mean(captured) > 0.1
+
+
+def test_nan_weights_take_the_plain_solve(gpu):
+    """Corrupted (NaN) network weights give a non-finite energy scale: the
+    learned step is rejected and the solve is the plain zero-guess GMG-PCG,
+    with the same iterations, residual and solution bit for bit."""
+    import torch
+    from topoptlab_amd import learned
+    from topoptlab_amd.elements import lk_linear_elast_3d
+    from topoptlab_amd.fem import StiffnessOperator
+    say("learned: NaN weights")
+    dims = (32, 16, 16)
+    held = []
+    learned.generate_pairs(*dims, runs=1, iters=3, seed=5, device=gpu, designs=held)
+    xPhys, f, fixed = held[0]
+    op = StiffnessOperator(*dims, KE=lk_linear_elast_3d(), fixed=fixed, device=gpu)
+    op.set_density(xPhys)
+    op.setup_mg(precision="fp32")
+    u0, i0 = op.solve(f, rtol=1e-8)
+    net = learned.CoarseNet(gpu, seed=0)
+    for bad in (np.nan, np.inf):
+        net.load(np.full(net.nparam, bad, dtype=np.float32))
+        u1, i1 = learned.solve_with_learned_guess(net, op, xPhys, f, rtol=1e-8)
+        assert not i1["learned"] and i1["alpha"] == 0.0
+        assert i1["iterations"] == i0["iterations"]
+        assert i1["relres_true"] == i0["relres_true"]
+        assert torch.equal(u1, u0)
+    op.close()

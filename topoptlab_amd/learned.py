@@ -288,6 +288,11 @@ def generate_pairs(nelx, nely, nelz, runs, iters, seed=0, volfrac=0.3, rmin=1.5,
     """(features, target) pairs from plain GMG-PCG solves along OC runs of
     cantilevers with seeded load positions (density filter, OC, move 0.2).
     With a list ``designs``, each run's last (xPhys, f, fixed) is appended."""
+    return _oc_runs(nelx, nely, nelz, runs, iters, seed, volfrac, rmin, device, log, designs,
+                    True)
+
+
+def _oc_runs(nelx, nely, nelz, runs, iters, seed, volfrac, rmin, device, log, designs, pairs):
     from .elements import lk_linear_elast_3d
     from .fem import StiffnessOperator
     from .filter import DeviceFilter
@@ -309,8 +314,9 @@ def generate_pairs(nelx, nely, nelz, runs, iters, seed=0, volfrac=0.3, rmin=1.5,
             op.set_density(xPhys)
             op.setup_mg(precision="fp32")
             u, info = op.solve(f, u0=u, rtol=1e-6)
-            X.append(features(op, xPhys, f))
-            T.append(target(op, u))
+            if pairs:
+                X.append(features(op, xPhys, f))
+                T.append(target(op, u))
             obj, dc = op.compliance_sens(u, xPhys)
             dc = flt.apply(dc, mode=1)
             dvf = flt.apply(dv, mode=1)
@@ -322,6 +328,15 @@ def generate_pairs(nelx, nely, nelz, runs, iters, seed=0, volfrac=0.3, rmin=1.5,
         op.close()
     flt.close()
     return X, T
+
+
+def oc_design(nelx, nely, nelz, seed=0, iters=8, volfrac=0.3, rmin=1.5, device="cuda"):
+    """A design of the training distribution at any size: ``iters`` OC
+    iterations (density filter, move 0.2) of a cantilever with a seeded point
+    load, as ``generate_pairs`` runs them.  Returns (xPhys, f, fixed)."""
+    designs = []
+    _oc_runs(nelx, nely, nelz, 1, iters, seed, volfrac, rmin, device, None, designs, False)
+    return designs[0]
 
 
 def generate_field_pairs(nelx, nely, nelz, count, seed=0, device="cuda", log=None):
