@@ -258,3 +258,28 @@ def test_main_log_file_format(gpu, tmp_path):
     assert np.allclose(hist["obj"], [o for o, _, _ in h], rtol=0, atol=1e-10 * 1e3)
     assert np.allclose(hist["vol"], [v for _, v, _ in h], rtol=0, atol=1e-10)
     assert np.allclose(hist["ch"], [c for _, _, c in h], rtol=0, atol=1e-10)
+
+
+def test_main_vtk_export_device_path(gpu, tmp_path):
+    """main(output_kw={"export": True}) after a device OC loop writes the
+    VTK file of output_designs.py:36-179: read back by the independent parser
+    of tests/test_vtk_export.py, x is the returned design, xPhys its density
+    filter (the oracle's H x / Hs to 1e-13), f the MBB load and u the last
+    device displacement (zero on the fixed DOFs)."""
+    from tests.test_vtk_export import read_vtk
+    from topoptlab_amd.topology_optimization import main
+    say("main VTK export")
+    base = str(tmp_path / "mbb")
+    nelx, nely = 30, 10
+    x, _ = main(nelx, nely, 0.5, 2.4, ft=1, nouteriter=6, stop_on_change=False, device=gpu,
+                output_kw={"file": base, "export": True, "write_log": False, "display": False})
+    r = read_vtk(base + ".vtk")
+    assert np.array_equal(r["cell_data"]["x"][:, 0], x[:, 0])
+    H, Hs = fem.assemble_matrix_filter(nelx, nely, 2.4)
+    assert rel(r["cell_data"]["xPhys"][:, 0], (H @ x[:, 0]) / Hs[:, 0]) < 1e-13
+    nn = (nelx + 1) * (nely + 1)
+    f, fixed, free = fem.mbb_2d(nelx, nely, 2 * nn)
+    assert np.array_equal(r["point_data"]["f0"].reshape(-1), f[:, 0])
+    u = r["point_data"]["u0"].reshape(-1)
+    assert np.all(u[fixed] == 0) and np.abs(u[free]).max() > 0
+    assert r["cell_data"]["xThresh"].sum() == np.floor(0.5 * nelx * nely)
