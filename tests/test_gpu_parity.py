@@ -121,11 +121,9 @@ def test_pcg_gmg_fp64_matches_oracle(gpu, dims, ndpn, nlevels):
 
 def test_pcg_gmg_fp64_matches_oracle_large_level(gpu):
     """128x64x64 H8 heat conduction (545k DOF), 4 levels: level 1 has 70,785
-    nodes, above the 65,536 up to which the coarse stencil kernel runs its
-    unrolled small-level form, so this case pins the looped form (the form
-    of the 192x96x96 workload's level 1, there with 3 DOFs per node; the
-    elasticity oracle at this node count takes minutes) against the oracle's
-    V-cycle; levels 2 and 3 take the unrolled form."""
+    nodes (3,639 workgroups of the grouped 3D stencil form, more than the
+    device holds at once) with one DOF per node (the 3-DOF case is the
+    fixture test below), against the oracle's V-cycle."""
     import torch
     from topoptlab_amd.fem import StiffnessOperator
     dims, ndpn, nlevels = (128, 64, 64), 1, 4
@@ -145,6 +143,37 @@ def test_pcg_gmg_fp64_matches_oracle_large_level(gpu):
     u = u[op._free_t].cpu().numpy()
     assert abs(inf["iterations"] - it_ref) <= 1, (inf, it_ref)
     assert rel(u, u_ref) < 1e-6
+    assert inf["relres_true"] < 2e-8
+    op.close()
+
+
+def test_pcg_gmg_fp64_large_level_elasticity_fixture(gpu):
+    """128x64x64 H8 elasticity, 4 levels: level 1 has 70,785 nodes with 3
+    DOFs each, a size class of the 192x96x96 workload's level 1 (232,897
+    nodes) that the small cases do not reach.  Against the oracle's FP64
+    GMG-PCG solution committed by tests/golden/make_gmg_large_level_fixture.py
+    (about 90 s of CPU): the same iteration count (+-1), u at 4096 seeded
+    free DOFs to 1e-6, b.u to 1e-7, both at relres 1e-8."""
+    import hashlib
+    import torch
+    from tests.conftest import GOLDEN
+    from topoptlab_amd.fem import StiffnessOperator
+    fx = np.load(GOLDEN / "gmg_elast_128x64x64_seed5.npz")
+    dims = tuple(int(v) for v in fx["dims"])
+    say("gmg fp64 large level elasticity fixture", dims)
+    KE, _, _, fixed, free, _ = _problem(dims, 3)
+    x = workloads.gaussian_random_density(*dims, seed=5, sigma=2.0)
+    assert hashlib.sha256(x.tobytes()).hexdigest() == str(fx["density_sha256"])
+    b = np.random.default_rng(3).standard_normal(len(free))
+    op = StiffnessOperator(*dims, KE=KE, fixed=fixed, device=gpu)
+    op.set_density(x)
+    op.setup_mg(nlevels=int(fx["nlevels"]), precision="fp64")
+    u, inf = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-8)
+    u = u[op._free_t].cpu().numpy()
+    say("iterations", inf["iterations"], "oracle", int(fx["iterations"]))
+    assert abs(inf["iterations"] - int(fx["iterations"])) <= 1
+    assert rel(u[fx["idx_free"]], fx["u_idx"]) < 1e-6
+    assert abs(b @ u - float(fx["bu"])) < 1e-7 * abs(float(fx["bu"]))
     assert inf["relres_true"] < 2e-8
     op.close()
 

@@ -1937,15 +1937,14 @@ void swap_sys(tl_problem* P, SysBufs& S) {
 }
 
 // Byte sizes of a system's buffers on handle P (same as the handle's own).
-std::vector<size_t> sys_sizes(tl_problem* P) {
-  std::vector<size_t> sz;
+void sys_sizes(tl_problem* P, std::vector<size_t>& sz) {
+  sz.clear();
   const size_t v = sizeof(double) * size_t(P->ndof);
   for (int k = 0; k < 7; ++k) sz.push_back(v);
   sz.push_back(sizeof(double) * 4 * kMaxRedBlocks);
   sz.push_back(sizeof(PcgState));
   for (int l = 0; l < P->mg.nlevels; ++l)
     for (int k = 0; k < 4; ++k) sz.push_back(std::max<size_t>(lv_slot(P, l, k)->bytes, 8));
-  return sz;
 }
 
 struct MergedLaunch {
@@ -2060,7 +2059,7 @@ int batch_lockstep(int32_t n, tl_problem* const* hs, const double* const* f, dou
   size_t total = 0;
   std::vector<std::vector<size_t>> sizes(n);
   for (int i = 0; i < n; ++i) {
-    sizes[i] = sys_sizes(hs[i]);
+    sys_sizes(hs[i], sizes[i]);
     for (int j = 0; j < nrhs[i]; ++j) {
       sys.push_back(Sys{i, j, {}, {}, {}});
       for (size_t b : sizes[i]) total += (b + 255) / 256 * 256;

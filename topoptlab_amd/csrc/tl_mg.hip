@@ -345,12 +345,133 @@ __device__ __forceinline__ void st_term(const GridD& g, const T* __restrict__ st
   }
 }
 
+// Raw buffer loads (gfx9 descriptor, no stride, range = nbytes): one uniform
+// base per array and a 32-bit per-lane byte offset, so a row's dozens of
+// loads in flight cost one address register per neighbour.
+template <typename T>
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t brsrc(const T* base, uint32_t nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), short(0), int(nbytes), 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+  static_assert(sizeof(T) == 8 || sizeof(T) == 4, "8- or 4-byte values");
+  if constexpr (sizeof(T) == 8)
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+  else
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+
+// Loads of 3D stencil term K (offset (K/9 - 1, K/3 % 3 - 1, K % 3 - 1) in
+// (z, x, y)) at node n: x of the neighbour and the block A(n, m), the forward
+// block of n for K >= 13, else the transposed forward block 26 - K of m.
+// Each forward-half slab kk (NDPN^2 arrays of nnodes values) is one buffer,
+// which the host keeps below 4 GiB (st_dimu).
+template <int NDPN, typename T, int K>
+__device__ __forceinline__ void st_load_term(const GridD& g, const T* __restrict__ sth,
+                                             __amdgpu_buffer_rsrc_t rxb, uint32_t n, int ix,
+                                             int iy, int iz, T (&xm)[NDPN],
+                                             T (&w)[NDPN * NDPN]) {
+  constexpr int NN = NDPN * NDPN;
+  constexpr int rz = K / 9 - 1, rx = K / 3 % 3 - 1, ry = K % 3 - 1;
+  const uint32_t arr = uint32_t(g.nnodes) * uint32_t(sizeof(T));
+  const uint32_t m = (uint32_t(min(max(iz + rz, 0), g.nz - 1)) * uint32_t(g.nx) +
+                      uint32_t(min(max(ix + rx, 0), g.nx - 1))) * uint32_t(g.ny) +
+                     uint32_t(min(max(iy + ry, 0), g.ny - 1));
+#pragma unroll
+  for (int c = 0; c < NDPN; ++c)
+    xm[c] = bload<T>(rxb, m * uint32_t(NDPN * sizeof(T)), uint32_t(c * sizeof(T)));
+  constexpr int kk = K >= 13 ? K - 13 : 13 - K;
+  const __amdgpu_buffer_rsrc_t r = brsrc(sth + int64_t(kk) * NN * g.nnodes, uint32_t(NN) * arr);
+  if constexpr (K >= 13) {
+#pragma unroll
+    for (int e = 0; e < NN; ++e) w[e] = bload<T>(r, n * uint32_t(sizeof(T)), uint32_t(e) * arr);
+  } else {
+    const bool ok = node_in(g, ix + rx, iy + ry, iz + rz);
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+      for (int c = 0; c < NDPN; ++c) {
+        const T v = bload<T>(r, m * uint32_t(sizeof(T)), uint32_t(c * NDPN + d) * arr);
+        w[d * NDPN + c] = ok ? v : T(0);
+      }
+  }
+}
+
+// A group of 3D stencil terms K...: the registers of their loads and their
+// sum into acc in the order given.
+template <int NDPN, typename T, int... K>
+struct StGroup {
+  static constexpr int NT = sizeof...(K), NN = NDPN * NDPN;
+  T xm[NT][NDPN], w[NT][NN];
+  __device__ __forceinline__ void load(const GridD& g, const T* __restrict__ sth,
+                                       __amdgpu_buffer_rsrc_t rxb, uint32_t n, int ix, int iy,
+                                       int iz) {
+    int j = 0;
+    ((st_load_term<NDPN, T, K>(g, sth, rxb, n, ix, iy, iz, xm[j], w[j]), ++j), ...);
+  }
+  __device__ __forceinline__ void sum(T (&acc)[NDPN]) const {
+#pragma unroll
+    for (int j = 0; j < NT; ++j)
+#pragma unroll
+      for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+        for (int c = 0; c < NDPN; ++c) acc[d] += w[j][d * NDPN + c] * xm[j][c];
+  }
+};
+
+// Loads of a group, then its sum.  The scheduling barrier keeps the FMAs
+// behind the group's loads (the scheduler would otherwise interleave them one
+// term at a time, leaving few loads in flight); the empty asm takes the sums
+// (the FMAs stay before it) and may read and write through the kernel's
+// pointers (the next group's loads stay after it), so that one group of
+// registers is live at a time.
+template <int NDPN, typename T, int... K>
+__device__ __forceinline__ void st_group(const GridD& g, const T* __restrict__ sth,
+                                         const T* __restrict__ x, __amdgpu_buffer_rsrc_t rxb,
+                                         uint32_t n, int ix, int iy, int iz, T (&acc)[NDPN]) {
+  StGroup<NDPN, T, K...> grp;
+  grp.load(g, sth, rxb, n, ix, iy, iz);
+  __builtin_amdgcn_sched_barrier(0);
+  grp.sum(acc);
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) asm volatile("" : "+v"(acc[d]));
+  asm volatile("" : : "s"(sth), "s"(x) : "memory");
+}
+
+// The 27 terms of node n's row of the 3D stencil, in pairs (K, 26 - K) and
+// then the centre: the backward block of a pair is the forward block that
+// its owner (a node of the plane below, or of the line or node before) reads
+// in the same group, which keeps the two reads close in time and in the L2
+// (1.38x the algorithmic bytes fetched instead of 1.62x in plane order).
+template <int NDPN, typename T>
+__device__ __forceinline__ void st_row_3d(const GridD& g, const T* __restrict__ sth,
+                                          const T* __restrict__ x, uint32_t n, int ix, int iy,
+                                          int iz, T (&acc)[NDPN]) {
+  const __amdgpu_buffer_rsrc_t rxb = brsrc(x, uint32_t(g.nnodes) * NDPN * uint32_t(sizeof(T)));
+  st_group<NDPN, T, 0, 26>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 1, 25>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 2, 24>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 3, 23>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 4, 22>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 5, 21>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 6, 20>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 7, 19>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 8, 18>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 9, 17>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 10, 16>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 11, 15>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 12, 14>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  st_group<NDPN, T, 13>(g, sth, x, rxb, n, ix, iy, iz, acc);
+}
+
 // MODE 0: y = b - A x; 1: y = x + omega D^-1 (b - A x); fixed DOFs of y are 0.
-// DIMU 0: runtime dimension, a loop over the planes (large levels: fewer
-// registers, two waves per SIMD); 2 or 3: every term unrolled, so the
+// DIMU 0: runtime dimension, a loop over the planes (large 2D levels: fewer
+// registers, two waves per SIMD); 2: 2D with every term unrolled, so the
 // forward/backward choice is resolved at compile time and the loads of all
-// terms are in flight together (small levels, whose few waves cannot hide a
-// chain of dependent loads).  Same terms in the same order either way.
+// terms are in flight together (small 2D levels, whose few waves cannot hide
+// a chain of dependent loads); 4: 3D, groups of terms whose loads are issued
+// together (st_row_3d), summed in the groups' order.  Forms 0 and 2 sum the
+// same terms in the same order.
 // PRO (MODE 1, DIMU 2): x is the iterate before the coarse correction cc,
 // which is added on the fly (prolongation and first post-smoothing sweep in
 // one pass, with the values the two separate passes give).
@@ -367,7 +488,10 @@ __device__ __forceinline__ void st_apply_node(const GridD& g, const T* __restric
   T acc[NDPN];
 #pragma unroll
   for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
-  if constexpr (DIMU == 0) {
+  if constexpr (DIMU == 4) {
+    static_assert(!PRO, "the fused correction is a 2D form");
+    st_row_3d<NDPN, T>(g, sth, x, uint32_t(n), ix, iy, iz, acc);
+  } else if constexpr (DIMU == 0) {
     const bool d3 = g.dim == 3;
     for (int rz = (d3 ? -1 : 0); rz <= (d3 ? 1 : 0); ++rz)
 #pragma unroll
@@ -412,7 +536,7 @@ __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restri
   const unsigned nb = gridDim.x, hw = blockIdx.x;
   const unsigned xcd = hw & 7u, q = nb >> 3, r = nb & 7u;
   const unsigned logical = xcd * q + min(xcd, r) + (hw >> 3);
-  const int64_t n = int64_t(logical) * BS + threadIdx.x;
+  const int64_t n = int64_t(logical) * blockDim.x + threadIdx.x;
   if (n >= g.nnodes) return;
   st_apply_node<NDPN, T, MODE, DIMU, PRO>(g, sth, x, b, dinv, omega, mask, y, n, cc);
 }
@@ -1063,16 +1187,24 @@ int restrict_to(tl_problem* P, int l, const Grid& gf, const T* rf, const int* do
                                   done);
 }
 
-// Levels of up to 64k nodes (a few hundred workgroups) take the unrolled
-// stencil form (2 or 3: the dimension; 0: the looped form).  NDPN 2 only
-// occurs in 2D and NDPN 3 in 3D for the unrolled forms.
+// Stencil form of a level: 3D levels take the grouped form (4) while each
+// forward-half slab (NDPN^2 FP64 arrays) fits one buffer (below 4 GiB, i.e.
+// up to 59 million nodes), 2D levels of up to 64k nodes (a few hundred
+// workgroups) the unrolled form (2), the rest the looped form (0).  NDPN 2
+// only occurs in 2D and NDPN 3 in 3D for the dimension-specific forms.
 template <int NDPN>
 int st_dimu(const Grid& g) {
-  if (g.nnodes > 65536) return 0;
-  if (g.dim == 3 && NDPN != 2) return 3;
-  if (g.dim == 2 && NDPN != 3) return 2;
+  if (g.dim == 3 && NDPN != 2)
+    return g.nnodes * NDPN * NDPN * 8 < (int64_t(1) << 32) ? 4 : 0;
+  if (g.dim == 2 && NDPN != 3 && g.nnodes <= 65536) return 2;
   return 0;
 }
+
+// Workgroup size of the grouped 3D form: 64 threads spread a level over 4x
+// as many workgroups, which balances them across the 256 CUs (level 1 of the
+// 192x96x96 problem: 3,639 workgroups instead of 910, residual 73 -> 69 us;
+// levels 2-3: 120-480 workgroups instead of 17-120).
+constexpr unsigned kStBlock3d = 64;
 
 template <int NDPN, typename T>
 int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const int* done) {
@@ -1100,15 +1232,17 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
                   StArgs<T>{gd, st, x, b, dinv, om, mk, y, done, CoarseCorr<T>{}}, nb, 0);
       return;
     }
-    hipLaunchKernelGGL((st_apply_kernel<NDPN, T, MODE_, DIM_>), dim3(nb), dim3(BS), 0, P->stream,
-                       gd, st, x, b, dinv, om, mk, y, done, CoarseCorr<T>{});
+    const unsigned bs = DIM_ == 4 ? kStBlock3d : unsigned(BS);
+    hipLaunchKernelGGL((st_apply_kernel<NDPN, T, MODE_, DIM_>),
+                       dim3(unsigned((L.g.nnodes + bs - 1) / bs)), dim3(bs), 0, P->stream, gd, st,
+                       x, b, dinv, om, mk, y, done, CoarseCorr<T>{});
   };
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
-  using I3 = std::integral_constant<int, 3>;
-  if (dimu == 3) {
-    if constexpr (NDPN != 2) { if (mode == 0) go(I0{}, I3{}); else go(I1{}, I3{}); }
+  using I4 = std::integral_constant<int, 4>;
+  if (dimu == 4) {
+    if constexpr (NDPN != 2) { if (mode == 0) go(I0{}, I4{}); else go(I1{}, I4{}); }
   } else if (dimu == 2) {
     if constexpr (NDPN != 3) { if (mode == 0) go(I0{}, I2{}); else go(I1{}, I2{}); }
   } else {
