@@ -293,12 +293,15 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True, lockstep_
         f"{fmt(ms_eager)}, one stream {fmt(ms_one_stream)}, one at a time {fmt(ms_seq)}); "
         f"iterations {min(its)}..{max(its)}, max true relres {max(rr):.2e}")
     out = {
-        "workload": "32 Q4 elasticity grids (nelx 128..1024, nely 64..512, multiples of 32, "
+        "workload": "32 Q4 elasticity grids (nelx 128..1024, nely 64..512 drawn, each rounded "
+                    "up to the multiple of 2^h its own h-halving hierarchy needs, "
                     "4 point-load cases, left edge clamped, Emin 1e-9) + 8 Q4 heat grids "
                     "(kmin 1e-3, uniform source, left edge T=0); GRF densities seed 0; "
                     "FP64 PCG, FP32 GMG V(1,1) (coarsest grid <= 3000 DOF), relres 1e-8; "
                     "all 136 systems in lockstep, one launch per kernel and place in the cycle",
         "instances": len(ops), "systems": len(its), "dof_total": int(sum(op.ndof for op in ops)),
+        "dof_total_drawn": int(sum((2 if sp["kind"] == "elast" else 1) * (sp["drawn"][0] + 1)
+                                   * (sp["drawn"][1] + 1) for sp in specs)),
         "dof_rhs_total": int(dof_rhs),
         "ms_per_step": round(ms_step, 2), "ms_setup": round(ms_setup, 2),
         "ms_batch_solve": round(ms_solve, 2), "batch_solve_mode": "lockstep",

@@ -5,7 +5,8 @@ import ctypes
 import numpy as np
 
 from topoptlab_amd import _lib
-from topoptlab_amd.workloads import batched_instances
+from topoptlab_amd.fem import auto_levels
+from topoptlab_amd.workloads import batched_instances, hierarchy_halvings
 
 
 def test_batched_instances_shapes():
@@ -16,8 +17,16 @@ def test_batched_instances_shapes():
         assert np.array_equal(x.get("load_nodes", []), y.get("load_nodes", []))
     assert [d["kind"] for d in a] == ["elast"] * 32 + ["heat"] * 8
     for d in a:
-        assert 128 <= d["nelx"] <= 1024 and 64 <= d["nely"] <= 512
-        assert d["nelx"] % 32 == 0 and d["nely"] % 32 == 0
+        dx, dy = d["drawn"]
+        assert 128 <= dx <= 1024 and 64 <= dy <= 512
+        # rounded up to the multiple its own hierarchy needs, never below the draw
+        ndpn = 2 if d["kind"] == "elast" else 1
+        m = 1 << hierarchy_halvings(dx, dy, ndpn, 3000)
+        assert d["nelx"] % m == 0 and d["nely"] % m == 0
+        assert dx <= d["nelx"] < dx + m and dy <= d["nely"] < dy + m
+        levels = auto_levels(d["nelx"], d["nely"], None, ndpn, 3000)
+        f = 1 << (levels - 1)
+        assert levels >= 2 and ndpn * (d["nelx"] // f + 1) * (d["nely"] // f + 1) <= 3000
         if d["kind"] == "elast":
             nodes = d["load_nodes"]
             assert len(nodes) == 4 and len(set(nodes.tolist())) == 4

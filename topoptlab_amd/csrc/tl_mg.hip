@@ -442,26 +442,39 @@ __device__ __forceinline__ void st_group(const GridD& g, const T* __restrict__ s
 // then the centre: the backward block of a pair is the forward block that
 // its owner (a node of the plane below, or of the line or node before) reads
 // in the same group, which keeps the two reads close in time and in the L2
-// (1.38x the algorithmic bytes fetched instead of 1.62x in plane order).
-template <int NDPN, typename T>
+// (1.37x the algorithmic bytes fetched instead of 1.62x in plane order).
+// GP pairs per group: 1 on large levels (few registers: every workgroup
+// resident at once), 3 on small ones (fewer dependent round trips per
+// thread: 5 groups instead of 14, for levels whose few waves leave the
+// memory latency exposed).
+template <int NDPN, typename T, int GP>
 __device__ __forceinline__ void st_row_3d(const GridD& g, const T* __restrict__ sth,
                                           const T* __restrict__ x, uint32_t n, int ix, int iy,
                                           int iz, T (&acc)[NDPN]) {
+  static_assert(GP == 1 || GP == 3, "1 or 3 pairs per group");
   const __amdgpu_buffer_rsrc_t rxb = brsrc(x, uint32_t(g.nnodes) * NDPN * uint32_t(sizeof(T)));
-  st_group<NDPN, T, 0, 26>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 1, 25>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 2, 24>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 3, 23>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 4, 22>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 5, 21>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 6, 20>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 7, 19>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 8, 18>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 9, 17>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 10, 16>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 11, 15>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 12, 14>(g, sth, x, rxb, n, ix, iy, iz, acc);
-  st_group<NDPN, T, 13>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  if constexpr (GP == 1) {
+    st_group<NDPN, T, 0, 26>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 1, 25>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 2, 24>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 3, 23>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 4, 22>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 5, 21>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 6, 20>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 7, 19>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 8, 18>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 9, 17>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 10, 16>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 11, 15>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 12, 14>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 13>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  } else {
+    st_group<NDPN, T, 0, 26, 1, 25, 2, 24>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 3, 23, 4, 22, 5, 21>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 6, 20, 7, 19, 8, 18>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 9, 17, 10, 16, 11, 15>(g, sth, x, rxb, n, ix, iy, iz, acc);
+    st_group<NDPN, T, 12, 14, 13>(g, sth, x, rxb, n, ix, iy, iz, acc);
+  }
 }
 
 // MODE 0: y = b - A x; 1: y = x + omega D^-1 (b - A x); fixed DOFs of y are 0.
@@ -469,9 +482,9 @@ __device__ __forceinline__ void st_row_3d(const GridD& g, const T* __restrict__ 
 // registers, two waves per SIMD); 2: 2D with every term unrolled, so the
 // forward/backward choice is resolved at compile time and the loads of all
 // terms are in flight together (small 2D levels, whose few waves cannot hide
-// a chain of dependent loads); 4: 3D, groups of terms whose loads are issued
-// together (st_row_3d), summed in the groups' order.  Forms 0 and 2 sum the
-// same terms in the same order.
+// a chain of dependent loads); 4 and 5: 3D, groups of terms whose loads are
+// issued together (st_row_3d with 1 or 3 pairs per group), summed in the
+// groups' order.  Forms 0 and 2 sum the same terms in the same order.
 // PRO (MODE 1, DIMU 2): x is the iterate before the coarse correction cc,
 // which is added on the fly (prolongation and first post-smoothing sweep in
 // one pass, with the values the two separate passes give).
@@ -488,9 +501,9 @@ __device__ __forceinline__ void st_apply_node(const GridD& g, const T* __restric
   T acc[NDPN];
 #pragma unroll
   for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
-  if constexpr (DIMU == 4) {
+  if constexpr (DIMU == 4 || DIMU == 5) {
     static_assert(!PRO, "the fused correction is a 2D form");
-    st_row_3d<NDPN, T>(g, sth, x, uint32_t(n), ix, iy, iz, acc);
+    st_row_3d<NDPN, T, DIMU == 4 ? 1 : 3>(g, sth, x, uint32_t(n), ix, iy, iz, acc);
   } else if constexpr (DIMU == 0) {
     const bool d3 = g.dim == 3;
     for (int rz = (d3 ? -1 : 0); rz <= (d3 ? 1 : 0); ++rz)
@@ -1187,15 +1200,18 @@ int restrict_to(tl_problem* P, int l, const Grid& gf, const T* rf, const int* do
                                   done);
 }
 
-// Stencil form of a level: 3D levels take the grouped form (4) while each
+// Stencil form of a level: 3D levels take the grouped form while each
 // forward-half slab (NDPN^2 FP64 arrays) fits one buffer (below 4 GiB, i.e.
-// up to 59 million nodes), 2D levels of up to 64k nodes (a few hundred
-// workgroups) the unrolled form (2), the rest the looped form (0).  NDPN 2
+// up to 59 million nodes), with 1 pair per group above 64k nodes (4) and 3
+// pairs up to 64k (5); 2D levels of up to 64k nodes (a few hundred
+// workgroups) take the unrolled form (2), the rest the looped form (0).  NDPN 2
 // only occurs in 2D and NDPN 3 in 3D for the dimension-specific forms.
 template <int NDPN>
 int st_dimu(const Grid& g) {
-  if (g.dim == 3 && NDPN != 2)
-    return g.nnodes * NDPN * NDPN * 8 < (int64_t(1) << 32) ? 4 : 0;
+  if (g.dim == 3 && NDPN != 2) {
+    if (g.nnodes * NDPN * NDPN * 8 >= (int64_t(1) << 32)) return 0;
+    return g.nnodes > 65536 ? 4 : 5;
+  }
   if (g.dim == 2 && NDPN != 3 && g.nnodes <= 65536) return 2;
   return 0;
 }
@@ -1232,7 +1248,7 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
                   StArgs<T>{gd, st, x, b, dinv, om, mk, y, done, CoarseCorr<T>{}}, nb, 0);
       return;
     }
-    const unsigned bs = DIM_ == 4 ? kStBlock3d : unsigned(BS);
+    const unsigned bs = DIM_ >= 4 ? kStBlock3d : unsigned(BS);
     hipLaunchKernelGGL((st_apply_kernel<NDPN, T, MODE_, DIM_>),
                        dim3(unsigned((L.g.nnodes + bs - 1) / bs)), dim3(bs), 0, P->stream, gd, st,
                        x, b, dinv, om, mk, y, done, CoarseCorr<T>{});
@@ -1241,8 +1257,11 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
   using I4 = std::integral_constant<int, 4>;
+  using I5 = std::integral_constant<int, 5>;
   if (dimu == 4) {
     if constexpr (NDPN != 2) { if (mode == 0) go(I0{}, I4{}); else go(I1{}, I4{}); }
+  } else if (dimu == 5) {
+    if constexpr (NDPN != 2) { if (mode == 0) go(I0{}, I5{}); else go(I1{}, I5{}); }
   } else if (dimu == 2) {
     if constexpr (NDPN != 3) { if (mode == 0) go(I0{}, I2{}); else go(I1{}, I2{}); }
   } else {

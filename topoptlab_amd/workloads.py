@@ -60,21 +60,40 @@ def seeded_loads_2d(rng, nelx, nely, ncases):
     return np.array(nodes, dtype=np.int64)
 
 
+def hierarchy_halvings(nelx, nely, ndpn, max_coarse_dof=3000):
+    """Fewest grid halvings (at least one) after which a 2D grid of about
+    nelx x nely elements, each count rounded up to a multiple of 2^h, has a
+    coarsest grid of at most ``max_coarse_dof`` DOFs."""
+    h = 1
+    while True:
+        m = 1 << h
+        cx, cy = -(-nelx // m), -(-nely // m)
+        if ndpn * (cx + 1) * (cy + 1) <= max_coarse_dof or min(cx, cy) < 2:
+            return h
+        h += 1
+
+
 def batched_instances(seed=0, n_elast=32, n_heat=8, xrange=(128, 1024), yrange=(64, 512),
-                      ncases=4, multiple=32):
+                      ncases=4, max_coarse_dof=3000):
     """Instances of the batched uneven-grid workload: ``n_elast`` 2D Q4
     elasticity grids (left edge clamped, ``ncases`` unit downward point loads
     at seeded boundary nodes) and ``n_heat`` 2D heat-conduction grids (uniform
     source, T = 0 on the left edge).  Element counts are drawn uniformly from
-    the ranges and rounded down to a multiple of ``multiple`` so that every
-    grid halves exactly into a multigrid hierarchy.  Returns dicts with kind,
-    nelx, nely, density_seed and (elasticity) load_nodes."""
+    the ranges; each instance's counts are then rounded up to the multiple of
+    2^h that its own multigrid hierarchy needs (h halvings down to a coarsest
+    grid of at most ``max_coarse_dof`` DOFs), so a grid is never smaller
+    than its draw and at most 2^h - 1 elements larger.  Returns dicts with
+    kind, nelx, nely, drawn (the drawn counts), density_seed and
+    (elasticity) load_nodes."""
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n_elast + n_heat):
-        nelx = max(multiple, int(rng.integers(xrange[0], xrange[1] + 1)) // multiple * multiple)
-        nely = max(multiple, int(rng.integers(yrange[0], yrange[1] + 1)) // multiple * multiple)
-        d = {"kind": "elast" if i < n_elast else "heat", "nelx": nelx, "nely": nely,
+        kind = "elast" if i < n_elast else "heat"
+        dx = int(rng.integers(xrange[0], xrange[1] + 1))
+        dy = int(rng.integers(yrange[0], yrange[1] + 1))
+        m = 1 << hierarchy_halvings(dx, dy, 2 if kind == "elast" else 1, max_coarse_dof)
+        nelx, nely = -(-dx // m) * m, -(-dy // m) * m
+        d = {"kind": kind, "nelx": nelx, "nely": nely, "drawn": (dx, dy),
              "density_seed": int(rng.integers(0, 2 ** 31))}
         if d["kind"] == "elast":
             d["load_nodes"] = seeded_loads_2d(rng, nelx, nely, ncases)
