@@ -507,7 +507,8 @@ __device__ inline void ku3d_segment(const KuArgs& a, const PlaneBufs<T>& pb, uns
   const int64_t j = j0 + threadIdx.x;
   const bool inplane = j < a.plane;
   const int64_t jc = inplane ? j : a.plane - 1;
-  const int ix = int(jc / a.ny), iy = int(jc % a.ny);
+  // 32-bit division (planes hold fewer than 2^31 nodes, tl_create)
+  const int ix = int(uint32_t(jc) / uint32_t(a.ny)), iy = int(jc) - ix * a.ny;
   T U[2][9][NDPN];
   T S[4];
   T carry[NDPN];
@@ -592,7 +593,8 @@ __device__ __forceinline__ void ku2d_body(const KuArgs& a, unsigned bid, T* lds)
   const int64_t j = j0 + threadIdx.x;
   const bool active = j < a.plane;
   const int64_t jc = active ? j : a.plane - 1;
-  const int ix = int(jc / a.ny), iy = int(jc % a.ny);
+  // 32-bit division (planes hold fewer than 2^31 nodes, tl_create)
+  const int ix = int(uint32_t(jc) / uint32_t(a.ny)), iy = int(jc) - ix * a.ny;
   T U[3][9][NDPN];
   T S[3][4];
   stage_plane<NDPN, T>(a, lds, 0, j0, FUSEP);
@@ -1021,9 +1023,10 @@ __global__ void diag_kernel(const double* __restrict__ sp, const double* __restr
   if (n >= nnodes) return;
   const int nd = (dim == 3 ? 8 : 4) * NDPN;
   const int plane = (nelx + 1) * ny;
-  const int iz = int(n / plane);
-  const int64_t j = n % plane;
-  const int ix = int(j / ny), iy = int(j % ny);
+  // 32-bit divisions (fewer than 2^31 nodes, tl_create)
+  const int iz = int(uint32_t(n) / uint32_t(plane));
+  const int j = int(n) - iz * plane;
+  const int ix = int(uint32_t(j) / uint32_t(ny)), iy = j - ix * ny;
   double acc[NDPN];
   for (int d = 0; d < NDPN; ++d) acc[d] = 0.0;
   for (int oz = (dim == 3 ? -1 : 0); oz <= 0; ++oz)

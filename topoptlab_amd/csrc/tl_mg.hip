@@ -41,11 +41,14 @@ GridD to_d(const Grid& g) {
   return GridD{g.dim, g.nelx, g.nely, g.nelz, g.nx, g.ny, g.nz, g.nnodes};
 }
 
+// 32-bit divisions (tl_create keeps nnodes * ndpn below 2^31): a 64-bit
+// division by a runtime divisor costs about a hundred instructions.
 __device__ inline void node_xyz(const GridD& g, int64_t n, int& ix, int& iy, int& iz) {
-  iy = int(n % g.ny);
-  const int64_t t = n / g.ny;
-  ix = int(t % g.nx);
-  iz = int(t / g.nx);
+  const uint32_t u = uint32_t(n), ny = uint32_t(g.ny), nx = uint32_t(g.nx);
+  const uint32_t t = u / ny;
+  iy = int(u - t * ny);
+  ix = int(t % nx);
+  iz = int(t / nx);
 }
 
 __device__ inline int64_t node_id(const GridD& g, int ix, int iy, int iz) {

@@ -49,14 +49,16 @@ struct FiltArgs {
 
 // out_e = sum_{j in box(e), inside} w_ej * v_j   (v == nullptr: v_j = 1)
 __device__ inline double box_sum(const FiltArgs& a, int64_t e, const double* __restrict__ v) {
+  // 32-bit divisions (fewer than 2^31 elements)
+  const uint32_t eu = uint32_t(e), ly = uint32_t(a.nely), lxy = uint32_t(a.nelx) * ly;
   int x, y, z;
   if (a.dim == 2) {
-    x = int(e / a.nely); y = int(e % a.nely); z = 0;
+    z = 0;
   } else {
-    z = int(e / (int64_t(a.nelx) * a.nely));
-    const int64_t rest = e % (int64_t(a.nelx) * a.nely);
-    x = int(rest / a.nely); y = int(rest % a.nely);
+    z = int(eu / lxy);
   }
+  const uint32_t rest = eu - uint32_t(z) * lxy;
+  x = int(rest / ly); y = int(rest - uint32_t(x) * ly);
   const int W = 2 * a.R + 1;
   const int zr = a.dim == 3 ? a.R : 0;
   double acc = 0.0;
@@ -122,14 +124,11 @@ __global__ __launch_bounds__(BS) void compliance_kernel(
   double objp = 0.0;
   for (int64_t e = blockIdx.x * int64_t(BS) + threadIdx.x; e < nel;
        e += int64_t(gridDim.x) * BS) {
-    int ex, ey, ez;
-    if (D3) {
-      ez = int(e / (int64_t(nelx) * nely));
-      const int64_t rest = e % (int64_t(nelx) * nely);
-      ex = int(rest / nely); ey = int(rest % nely);
-    } else {
-      ez = 0; ex = int(e / nely); ey = int(e % nely);
-    }
+    // 32-bit divisions (fewer than 2^31 elements)
+    const uint32_t eu = uint32_t(e), ly = uint32_t(nely), lxy = uint32_t(nelx) * ly;
+    const int ez = D3 ? int(eu / lxy) : 0;
+    const uint32_t rest = eu - uint32_t(ez) * lxy;
+    const int ex = int(rest / ly), ey = int(rest - uint32_t(ex) * ly);
     const int ny = nely + 1, nx = nelx + 1;
     const double xe = xphys[e];
     const double se = eps + (1.0 - eps) * pow(xe, penal);
@@ -408,6 +407,8 @@ int tl_filter_create(int32_t ndim, int32_t nelx, int32_t nely, int32_t nelz, dou
   TL_ARG(ndim == 2 || ndim == 3, "ndim must be 2 or 3");
   TL_ARG(nelx > 0 && nely > 0 && (ndim == 2 || nelz > 0), "element counts must be positive");
   TL_ARG(rmin > 0.0 && std::isfinite(rmin), "rmin must be positive");
+  TL_ARG(int64_t(nelx) * nely * (ndim == 3 ? nelz : 1) < (int64_t(1) << 31),
+         "grid too large for 32-bit element indexing");
   const int R = std::max(0, int(std::ceil(rmin)) - 1);
   const int W = 2 * R + 1;
   TL_ARG((ndim == 3 ? W * W * W : W * W) <= kMaxBox, "rmin too large for the filter stencil");
