@@ -515,10 +515,34 @@ def main():
                     help="skip the batched uneven-grid workload in the full run")
     ap.add_argument("--batched-lockstep-only", action="store_true",
                     help="with --batched: time only the lockstep batch solve")
+    ap.add_argument("--calibrate-read", action="store_true",
+                    help="time a 235 MB device read after 1 GB written, after 1 GB read and warm")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (5 launches) so that "
                          "FETCH_SIZE/WRITE_SIZE counters can be scaled to bytes")
     args = ap.parse_args()
+    if args.calibrate_read:
+        # A 235 MB FP64 read (the size of the level-1 stencil at 192x96x96)
+        # after 1 GB written (dirty MALL lines to write back), after 1 GB read
+        # (clean), and warm (read again at once).
+        import torch
+        a = torch.ones(29345022, dtype=torch.float64, device="cuda")
+        big = torch.empty(128 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        for name, prep in (("after 1 GB written", lambda: big.fill_(1.0)),
+                           ("after 1 GB read", lambda: big.sum()),
+                           ("warm", lambda: a.sum())):
+            best = float("inf")
+            for _ in range(5):
+                prep()
+                ev[0].record()
+                a.sum()
+                ev[1].record()
+                torch.cuda.synchronize()
+                best = min(best, ev[0].elapsed_time(ev[1]))
+            log(f"read calibration, {name}: {best * 1e3:.1f} us, "
+                f"{a.numel() * 8 / (best * 1e-3) / 1e12:.2f} TB/s for {a.numel() * 8 / 1e6:.1f} MB")
+        return
     if args.calibrate_copy:
         import torch
         a = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device="cuda")
