@@ -1060,11 +1060,17 @@ __global__ __launch_bounds__(BS) void gj_panel_kernel(double* A, int64_t np, int
   }
 }
 
+// Launch 2 on the FP64 matrix cores (v_mfma_f64_16x16x4f64): S = C X over
+// the 64x64 tile, wave w computing rows 16w..16w+15 as 4 blocks of 16x16,
+// GJB/4 K slices each.  Lane layouts: lane l holds A[l%16][l/16] and
+// B[l/16][l%16] of a K slice; output element i of lane l is C[l/16 + 4i][l%16].
+typedef double gj_d4 __attribute__((ext_vector_type(4)));
 __global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, int k,
                                                         const double* __restrict__ colsave,
                                                         const double* __restrict__ pinv) {
   // X[t][j] = new row panel A[K][j] for j outside K, P^-1[t][j - kb] inside K;
   // S = C X gives the trailing update (A -= S) and the column panel (A = -S)
+  static_assert(BS == 256 && GJB % 4 == 0, "four waves, K slices of 4");
   __shared__ double cp[64][GJB + 1];
   __shared__ double xp[GJB][64 + 1];
   const int64_t kb = int64_t(k) * GJB;
@@ -1079,35 +1085,34 @@ __global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, i
     xp[r2][c2] = x;
   }
   __syncthreads();
-  const int tj = threadIdx.x % 16, ti = threadIdx.x / 16;  // 16 x 16 threads, 4x4 each
-  double acc[4][4] = {};
-#pragma unroll 8
-  for (int t = 0; t < GJB; ++t) {
-    double av[4], bv[4];
+  const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
+  gj_d4 acc[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) { av[u] = cp[ti * 4 + u][t]; bv[u] = xp[t][tj * 4 + u]; }
+  for (int b = 0; b < 4; ++b) acc[b] = gj_d4{0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+  for (int sl = 0; sl < GJB / 4; ++sl) {
+    const double av = cp[16 * w + lr][4 * sl + lk];
 #pragma unroll
-      for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+    for (int b = 0; b < 4; ++b)
+      acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xp[4 * sl + lk][16 * b + lr], acc[b], 0, 0, 0);
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int64_t i = i0 + ti * 4 + u;
-    if (i >= np) continue;
-    const bool irow = i >= kb && i < kb + GJB;
+  for (int b = 0; b < 4; ++b) {
+    const int64_t j = j0 + 16 * b + lr;
+    if (j >= np) continue;
+    const bool jcol = j >= kb && j < kb + GJB;
 #pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int64_t j = j0 + tj * 4 + v;
-      if (j >= np) continue;
-      const bool jcol = j >= kb && j < kb + GJB;
+    for (int e = 0; e < 4; ++e) {
+      const int64_t i = i0 + 16 * w + lk + 4 * e;
+      if (i >= np) continue;
+      const bool irow = i >= kb && i < kb + GJB;
       if (irow) {
         if (jcol) A[i * np + j] = pinv[(i - kb) * GJB + (j - kb)];
         // rows in K outside the pivot columns: the new row panel, already written
       } else if (jcol) {
-        A[i * np + j] = -acc[u][v];
+        A[i * np + j] = -acc[b][e];
       } else {
-        A[i * np + j] -= acc[u][v];
+        A[i * np + j] -= acc[b][e];
       }
     }
   }
