@@ -994,12 +994,14 @@ __global__ void pad_identity_kernel(double* A, int64_t n, int64_t np) {
 }
 
 // Two-launch form of the blocked Gauss-Jordan step k (pivot block K =
-// [kb, kb+GJB)).  Launch 1 (one workgroup per 64 columns): every workgroup
-// inverts the pivot block in LDS (the same fixed arithmetic in each, so all
-// copies agree), writes the new row panel A[K][j] = P^-1 A[K][j] for its
-// columns j outside K and saves the old column panel of its 64 rows; the
-// first workgroup also saves P^-1.  Launch 2 (64x64 tiles): trailing update
-// A[i][j] -= C[i] R[j], column panel A[i][K] = -C[i] P^-1, pivot block P^-1.
+// [kb, kb+GJB)).  Launch 1 (one workgroup per 64 columns): writes the new row
+// panel A[K][j] = P^-1 A[K][j] for its columns j outside K and saves the old
+// column panel of its 64 rows; at step 0 every workgroup first inverts the
+// pivot block in LDS (the same fixed arithmetic in each, so all copies
+// agree) and the first one saves P^-1, later steps read the P^-1 that the
+// previous update launch formed.  Launch 2 (64x64 tiles): trailing update
+// A[i][j] -= C[i] R[j], column panel A[i][K] = -C[i] P^-1, pivot block P^-1,
+// and the inverse of the next pivot block.
 __device__ inline void gj_invert_pivot(double (*a)[GJB + 1]) {
   // in-place Gauss-Jordan inverse of the GJB x GJB LDS block, BS threads
   for (int t = 0; t < GJB; ++t) {
@@ -1029,12 +1031,14 @@ __device__ inline void gj_invert_pivot(double (*a)[GJB + 1]) {
 constexpr int GJ_COLS = 64;
 __global__ __launch_bounds__(BS) void gj_panel_kernel(double* A, int64_t np, int k,
                                                       double* __restrict__ colsave,
-                                                      double* __restrict__ pinv) {
+                                                      double* __restrict__ pinv, int given) {
   static_assert(BS == 4 * GJ_COLS && GJB % 4 == 0, "four waves of 64 columns");
   __shared__ double a[GJB][GJB + 1];
   const int64_t kb = int64_t(k) * GJB;
   const int64_t r0 = int64_t(blockIdx.x) * GJ_COLS;
-  for (int e = threadIdx.x; e < GJB * GJB; e += BS) a[e / GJB][e % GJB] = A[(kb + e / GJB) * np + kb + e % GJB];
+  // given: P^-1 was formed by the previous step's update launch
+  for (int e = threadIdx.x; e < GJB * GJB; e += BS)
+    a[e / GJB][e % GJB] = given ? pinv[e] : A[(kb + e / GJB) * np + kb + e % GJB];
   // old column panel of the workgroup's 64 rows, 256-byte segments
   for (int e = threadIdx.x; e < GJ_COLS * GJB; e += BS) {
     const int64_t row = r0 + e / GJB;
@@ -1046,9 +1050,11 @@ __global__ __launch_bounds__(BS) void gj_panel_kernel(double* A, int64_t np, int
 #pragma unroll
   for (int t = 0; t < GJB; ++t) col[t] = active ? A[(kb + t) * np + j] : 0.0;
   __syncthreads();  // every old value read before any row-panel write below
-  gj_invert_pivot(a);
-  if (blockIdx.x == 0)
-    for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv[e] = a[e / GJB][e % GJB];
+  if (!given) {
+    gj_invert_pivot(a);
+    if (blockIdx.x == 0)
+      for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv[e] = a[e / GJB][e % GJB];
+  }
   if (!active) return;
   const int rb = (threadIdx.x / GJ_COLS) * (GJB / 4);
 #pragma unroll
@@ -1064,17 +1070,28 @@ __global__ __launch_bounds__(BS) void gj_panel_kernel(double* A, int64_t np, int
 // the 64x64 tile, wave w computing rows 16w..16w+15 as 4 blocks of 16x16,
 // GJB/4 K slices each.  Lane layouts: lane l holds A[l%16][l/16] and
 // B[l/16][l%16] of a K slice; output element i of lane l is C[l/16 + 4i][l%16].
+// The workgroup whose tile holds the next pivot block (rows and columns
+// [kb + GJB, kb + 2 GJB), inside one 64x64 tile) runs first (hardware block
+// (0, 0)), keeps that block's updated values in LDS and inverts it while the
+// other tiles are updated: the next panel launch then starts from P^-1.
 typedef double gj_d4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, int k,
+__global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, int k, int nbk,
                                                         const double* __restrict__ colsave,
-                                                        const double* __restrict__ pinv) {
+                                                        const double* __restrict__ pinv,
+                                                        double* __restrict__ pinv_next) {
   // X[t][j] = new row panel A[K][j] for j outside K, P^-1[t][j - kb] inside K;
   // S = C X gives the trailing update (A -= S) and the column panel (A = -S)
   static_assert(BS == 256 && GJB % 4 == 0, "four waves, K slices of 4");
   __shared__ double cp[64][GJB + 1];
   __shared__ double xp[GJB][64 + 1];
-  const int64_t kb = int64_t(k) * GJB;
-  const int64_t i0 = int64_t(blockIdx.y) * 64, j0 = int64_t(blockIdx.x) * 64;
+  __shared__ double piv[GJB][GJB + 1];
+  const int64_t kb = int64_t(k) * GJB, kn = kb + GJB;
+  const unsigned tn = k + 1 < nbk ? unsigned(kn / 64) : ~0u;  // tile of the next pivot
+  unsigned bx = blockIdx.x, by = blockIdx.y;
+  if (bx == 0 && by == 0) { bx = tn == ~0u ? 0u : tn; by = bx; }
+  else if (bx == tn && by == tn) { bx = 0; by = 0; }
+  const bool own_next = tn != ~0u && bx == tn && by == tn;
+  const int64_t i0 = int64_t(by) * 64, j0 = int64_t(bx) * 64;
   for (int t = threadIdx.x; t < 64 * GJB; t += BS) {
     const int r = t / GJB, c = t % GJB;
     cp[r][c] = (i0 + r < np) ? colsave[(i0 + r) * GJB + c] : 0.0;
@@ -1112,10 +1129,17 @@ __global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, i
       } else if (jcol) {
         A[i * np + j] = -acc[b][e];
       } else {
-        A[i * np + j] -= acc[b][e];
+        const double v = A[i * np + j] - acc[b][e];
+        A[i * np + j] = v;
+        if (own_next && i >= kn && i < kn + GJB && j >= kn && j < kn + GJB)
+          piv[i - kn][j - kn] = v;
       }
     }
   }
+  if (!own_next) return;
+  __syncthreads();
+  gj_invert_pivot(piv);
+  for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv_next[e] = piv[e / GJB][e % GJB];
 }
 
 unsigned gsz(int64_t work) {
@@ -1619,14 +1643,14 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
     }
     const int nbk = int(M.cnp / GJB);
     const unsigned nt = unsigned((M.cnp + 63) / 64);
-    TL_TRY(M.gjw.alloc(sizeof(double) * (size_t(M.cnp) * GJB + GJB * GJB)));
+    TL_TRY(M.gjw.alloc(sizeof(double) * (size_t(M.cnp) * GJB + 2 * GJB * GJB)));
     double* colsave = M.gjw.as<double>();
-    double* pinv = colsave + size_t(M.cnp) * GJB;
+    double* pinv[2] = {colsave + size_t(M.cnp) * GJB, colsave + size_t(M.cnp) * GJB + GJB * GJB};
     for (int k = 0; k < nbk; ++k) {
       hipLaunchKernelGGL(gj_panel_kernel, dim3(unsigned((M.cnp + GJ_COLS - 1) / GJ_COLS)), dim3(BS), 0, s, A,
-                         M.cnp, k, colsave, pinv);
-      hipLaunchKernelGGL(gj_update2_kernel, dim3(nt, nt), dim3(BS), 0, s, A, M.cnp, k, colsave,
-                         pinv);
+                         M.cnp, k, colsave, pinv[k & 1], int(k > 0));
+      hipLaunchKernelGGL(gj_update2_kernel, dim3(nt, nt), dim3(BS), 0, s, A, M.cnp, k, nbk, colsave,
+                         pinv[k & 1], pinv[(k + 1) & 1]);
     }
     TL_CHECK_LAUNCH();
   }
