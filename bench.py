@@ -234,7 +234,7 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True, lockstep_
     PCG with FP32 V-cycles to relres 1e-8.  A step = set_density +
     setup_mg on every instance + one tl_batch_solve of all 136 systems."""
     import torch
-    from topoptlab_amd.batch import autotune, batch_solve, build_instance
+    from topoptlab_amd.batch import autotune, batch_setup, batch_solve, build_instance
     from topoptlab_amd.workloads import batched_instances
     specs = batched_instances(seed=0)
     t0 = time.perf_counter()
@@ -246,9 +246,7 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True, lockstep_
     log(f"batched: {len(ops)} instances built on the host in {time.perf_counter() - t0:.1f} s")
 
     def setup_all():
-        for op, x, lv in zip(ops, xs, levels):
-            op.set_density(x)
-            op.setup_mg(nlevels=lv, precision="fp32")
+        batch_setup(ops, xs, levels, precision="fp32", nstreams=nstreams)
 
     def solve_all(graphs=True, streams=nstreams, lockstep=True):
         return batch_solve(ops, fs, rtol=1e-8, nstreams=streams, chunk=chunk, use_graphs=graphs,

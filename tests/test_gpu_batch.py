@@ -109,11 +109,11 @@ def test_autotune_table(gpu):
 
 def test_batch_matches_single_solves(gpu):
     import torch
-    from topoptlab_amd.batch import autotune, batch_solve, build_instance
+    from topoptlab_amd.batch import autotune, batch_setup, batch_solve, build_instance
     from topoptlab_amd.workloads import batched_instances
     specs = batched_instances(seed=3, n_elast=4, n_heat=2, xrange=(64, 192), yrange=(32, 128),
                               ncases=3)
-    ops, fs, xs = [], [], []
+    ops, fs, xs, lvs = [], [], [], []
     for spec in specs:
         say("batch instance", spec["kind"], spec["nelx"], spec["nely"])
         op, x, f, levels = build_instance(spec, gpu)
@@ -122,8 +122,14 @@ def test_batch_matches_single_solves(gpu):
         autotune(op, reps=3)
         ops.append(op)
         xs.append(x)
+        lvs.append(levels)
         fs.append(torch.as_tensor(f, device=gpu))
     ref = [[op.solve(f[k], rtol=1e-8) for k in range(f.shape[0])] for op, f in zip(ops, fs)]
+    # the set-ups again, overlapped on 3 streams: the batch solves below must
+    # still equal the single solves after the one-after-another set-up
+    say("batch set-up on 3 streams")
+    batch_setup(ops, [torch.as_tensor(x, device=gpu) for x in xs], lvs, precision="fp32",
+                nstreams=3)
     # lockstep (one launch per kernel and place in the cycle for all systems),
     # then per-instance streams with and without captured graphs; each twice
     # in a row (the second call reuses what the first set up)

@@ -18,6 +18,8 @@ of uneven grid sizes is handed to ``tl_batch_solve`` at once:
   instance's chunk of PCG iterations is captured once into a hipGraph and
   replayed until the device-side state reports convergence, and the load
   cases of one instance run one after another on its stream;
+* ``batch_setup`` overlaps the instances' multigrid set-ups on several
+  streams;
 * ``autotune`` picks the 2D operator's workgroup shape per grid by timing.
 """
 from __future__ import annotations
@@ -55,6 +57,28 @@ def autotune(op: StiffnessOperator, reps: int = 20) -> Dict:
 
 def clear_autotune_table() -> None:
     lib().tl_autotune_clear()
+
+
+def batch_setup(ops: Sequence[StiffnessOperator], xs: Sequence, levels: Optional[Sequence[int]] = None,
+                precision: str = "fp32", nstreams: int = 16, **mg_kw) -> None:
+    """``set_density`` + ``setup_mg`` of every instance, issued on ``nstreams``
+    side streams in turn so that the instances' set-up kernels overlap (each
+    set-up is a chain of small dependent launches: the Galerkin coarsening and
+    the ~60-90 Gauss-Jordan steps of its coarsest inverse).  The current
+    stream waits for all of them before it returns; the results are those of
+    the one-after-another set-up."""
+    if len(xs) != len(ops) or (levels is not None and len(levels) != len(ops)):
+        raise ValueError("need one density (and level count) per operator")
+    main = torch.cuda.current_stream(ops[0].device) if ops else None
+    streams = [torch.cuda.Stream(device=op.device) for op in ops[:max(1, nstreams)]]
+    for s in streams:
+        s.wait_stream(main)  # densities produced on the current stream
+    for i, op in enumerate(ops):
+        with torch.cuda.stream(streams[i % len(streams)]):
+            op.set_density(xs[i])
+            op.setup_mg(nlevels=None if levels is None else levels[i], precision=precision, **mg_kw)
+    for s in streams:
+        main.wait_stream(s)
 
 
 def batch_solve(ops: Sequence[StiffnessOperator], fs: Sequence[torch.Tensor], rtol: float = 1e-8,
