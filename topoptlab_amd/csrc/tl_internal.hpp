@@ -191,7 +191,7 @@ struct MgState {
   DevBuf G1;              // 8 x (nd*nd) fp64: child Galerkin matrices
   DevBuf W;               // 8 x 8 fp64: child interpolation weights
   DevBuf cinv;            // dense inverse of the coarsest operator (fp64)
-  DevBuf gjw;             // Gauss-Jordan workspace: saved column panel, pivot inverse
+  DevBuf gjw;             // Gauss-Jordan workspace: row panels and pivot inverses of two steps
   int64_t cn = 0, cnp = 0;  // coarsest ndof, padded size
 };
 

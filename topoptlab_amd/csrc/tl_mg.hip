@@ -993,113 +993,174 @@ __global__ void pad_identity_kernel(double* A, int64_t n, int64_t np) {
   if (i < np) A[i * np + i] = 1.0;
 }
 
-// Two-launch form of the blocked Gauss-Jordan step k (pivot block K =
-// [kb, kb+GJB)).  Launch 1 (one workgroup per 64 columns): writes the new row
-// panel A[K][j] = P^-1 A[K][j] for its columns j outside K and saves the old
-// column panel of its 64 rows; at step 0 every workgroup first inverts the
-// pivot block in LDS (the same fixed arithmetic in each, so all copies
-// agree) and the first one saves P^-1, later steps read the P^-1 that the
-// previous update launch formed.  Launch 2 (64x64 tiles): trailing update
-// A[i][j] -= C[i] R[j], column panel A[i][K] = -C[i] P^-1, pivot block P^-1,
-// and the inverse of the next pivot block.
-__device__ inline void gj_invert_pivot(double (*a)[GJB + 1]) {
-  // in-place Gauss-Jordan inverse of the GJB x GJB LDS block, BS threads
-  for (int t = 0; t < GJB; ++t) {
-    double nv[GJB * GJB / BS];
-    const double inv = 1.0 / a[t][t];  // one reciprocal per step
+// Blocked Gauss-Jordan inverse of the symmetric positive definite coarsest
+// matrix, one launch per block step k (pivot block K = [kb, kb+GJB)):
+//   new row panel   A[K][j] = X[j] = P^-1 R[j]      (j outside K, R = old A[K][j])
+//   trailing update A[i][j] -= C[i] X[j]           (C = old A[i][K])
+//   column panel    A[i][K] = -C[i] P^-1, pivot block A[K][K] = P^-1.
+// Every intermediate satisfies A[i][j] = s(i) s(j) A[j][i] with s = -1 on the
+// indices already swept (below kb), so only the lower 64x64 tiles (tile row
+// >= tile column; diagonal tiles whole) are updated, and the column panel is
+// C[i][t] = s(i) R[t][i].  Step k reads R (the full row panel of K) and P^-1
+// from buffers that step k-1 wrote, so no tile reads a value that another
+// tile of the same launch writes.  Each tile forms its own columns of X and
+// writes its part of the next row panel: the tiles of the next pivot's tile
+// row their rows of it, the tiles below the next pivot tile their columns of
+// the next block, transposed (those rows are not swept, s = +1).  The tile
+// that holds the next pivot block inverts it.  After the last step the lower
+// tiles hold the (symmetric) inverse and are mirrored into the upper ones.
+
+// In-place Gauss-Jordan inverse of the GJB x GJB LDS block by wave 0; every
+// thread of the workgroup calls it.  Lane l keeps row l & 31, columns
+// 16 (l >> 5) .. + 15 in registers; each step publishes the pivot row and
+// column through `pub` (2 x 64 doubles).  One wave's LDS operations execute
+// in order, so the steps need no workgroup barrier.
+__device__ inline void gj_invert_pivot(double (*a)[GJB + 1], double* pub) {
+  static_assert(GJB == 32, "one wave: 32 rows x 2 column halves");
+  if (threadIdx.x < 64) {
+    const int r = threadIdx.x & 31, h = threadIdx.x >> 5;
+    double v[16];
 #pragma unroll
-    for (int u = 0; u < GJB * GJB / BS; ++u) {
-      const int e = threadIdx.x + u * BS, i = e / GJB, j = e % GJB;
-      const double ait = a[i][t], atj = a[t][j] * inv, aij = a[i][j];
-      if (i == t && j == t) nv[u] = inv;
-      else if (i == t) nv[u] = atj;
-      else if (j == t) nv[u] = -ait * inv;
-      else nv[u] = aij - ait * atj;
-    }
-    __syncthreads();
+    for (int c = 0; c < 16; ++c) v[c] = a[r][16 * h + c];
 #pragma unroll
-    for (int u = 0; u < GJB * GJB / BS; ++u) {
-      const int e = threadIdx.x + u * BS;
-      a[e / GJB][e % GJB] = nv[u];
+    for (int t = 0; t < GJB; ++t) {
+      double* rb = pub + (t & 1) * 64;  // row t
+      double* cb = rb + 32;             // column t
+      if (r == t) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) rb[16 * h + c] = v[c];
+      }
+      if (h == (t >> 4)) cb[r] = v[t & 15];
+      __builtin_amdgcn_s_waitcnt(0xc07f);  // lgkmcnt(0): the step's writes have landed
+      __builtin_amdgcn_wave_barrier();
+      const double att = rb[t], ait = cb[r];
+      double rt[16];
+#pragma unroll
+      for (int c = 0; c < 16; ++c) rt[c] = rb[16 * h + c];
+      const double inv = 1.0 / att;  // one reciprocal per step
+#pragma unroll
+      for (int c = 0; c < 16; ++c) {
+        const bool jt = h == (t >> 4) && c == (t & 15);
+        const double atj = rt[c] * inv;
+        double nv;
+        if (r == t && jt) nv = inv;
+        else if (r == t) nv = atj;
+        else if (jt) nv = -ait * inv;
+        else nv = v[c] - ait * atj;
+        v[c] = nv;
+      }
+      __builtin_amdgcn_wave_barrier();
     }
-    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < 16; ++c) a[r][16 * h + c] = v[c];
   }
+  __syncthreads();
 }
 
-// launch 1: workgroup b covers rows and columns [64b, 64b + 64); wave w
-// writes rows kb + 8w .. kb + 8w + 7 of the new row panel for its 64 columns
-constexpr int GJ_COLS = 64;
-__global__ __launch_bounds__(BS) void gj_panel_kernel(double* A, int64_t np, int k,
-                                                      double* __restrict__ colsave,
-                                                      double* __restrict__ pinv, int given) {
-  static_assert(BS == 4 * GJ_COLS && GJB % 4 == 0, "four waves of 64 columns");
+// Step-0 buffers: the row panel R = A[0..GJB)[j] for the workgroup's 64
+// columns; workgroup 0 also inverts the first pivot block.
+__global__ __launch_bounds__(BS) void gj_init_kernel(const double* __restrict__ A, int64_t np,
+                                                     double* __restrict__ rowp,
+                                                     double* __restrict__ pinv) {
   __shared__ double a[GJB][GJB + 1];
-  const int64_t kb = int64_t(k) * GJB;
-  const int64_t r0 = int64_t(blockIdx.x) * GJ_COLS;
-  // given: P^-1 was formed by the previous step's update launch
-  for (int e = threadIdx.x; e < GJB * GJB; e += BS)
-    a[e / GJB][e % GJB] = given ? pinv[e] : A[(kb + e / GJB) * np + kb + e % GJB];
-  // old column panel of the workgroup's 64 rows, 256-byte segments
-  for (int e = threadIdx.x; e < GJ_COLS * GJB; e += BS) {
-    const int64_t row = r0 + e / GJB;
-    if (row < np) colsave[row * GJB + e % GJB] = A[row * np + kb + e % GJB];
+  __shared__ double pub[128];
+  const int64_t c0 = int64_t(blockIdx.x) * 64;
+  for (int e = threadIdx.x; e < 64 * GJB; e += BS) {
+    const int64_t j = c0 + e % 64;
+    const int t = e / 64;
+    if (j < np) rowp[t * np + j] = A[t * np + j];
   }
-  const int64_t j = r0 + threadIdx.x % GJ_COLS;
-  const bool active = j < np && (j < kb || j >= kb + GJB);
+  if (blockIdx.x != 0) return;
+  for (int e = threadIdx.x; e < GJB * GJB; e += BS) a[e / GJB][e % GJB] = A[(e / GJB) * np + e % GJB];
+  __syncthreads();
+  gj_invert_pivot(a, pub);
+  for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv[e] = a[e / GJB][e % GJB];
+}
+
+// lower tile (I, J), I >= J, of linear index b = I (I + 1) / 2 + J
+__device__ inline void gj_tile_of(unsigned b, unsigned& I, unsigned& J) {
+  unsigned i = unsigned((sqrtf(8.0f * float(b) + 1.0f) - 1.0f) * 0.5f);
+  while ((i + 1) * (i + 2) / 2 <= b) ++i;
+  while (i * (i + 1) / 2 > b) --i;
+  I = i;
+  J = b - i * (i + 1) / 2;
+}
+
+// Step k on the lower 64x64 tiles.  S = C X on the FP64 matrix cores
+// (v_mfma_f64_16x16x4f64), wave w computing rows 16w..16w+15 as 4 blocks of
+// 16x16, GJB/4 K slices each.  Lane layouts: lane l holds A[l%16][l/16] and
+// B[l/16][l%16] of a K slice; output element i of lane l is C[l/16 + 4i][l%16].
+// The diagonal tile that holds the next pivot block (rows and columns
+// [kb + GJB, kb + 2 GJB)) runs first (hardware block 0) and inverts that
+// block while the other tiles are updated.
+typedef double gj_d4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(BS) void gj_step_kernel(double* A, int64_t np, int k, int nbk,
+                                                     const double* __restrict__ rowp,
+                                                     const double* __restrict__ pinv,
+                                                     double* __restrict__ rowp_next,
+                                                     double* __restrict__ pinv_next) {
+  static_assert(BS == 256 && GJB % 4 == 0 && GJB == 32, "four waves, K slices of 4");
+  __shared__ double cp[64][GJB + 1];   // C rows of the tile
+  __shared__ double xp[GJB][64 + 1];   // X columns of the tile
+  __shared__ double pb[GJB][GJB + 1];  // the next pivot block (its tile only)
+  __shared__ double pub[128];
+  const int64_t kb = int64_t(k) * GJB, kn = kb + GJB;
+  const bool has_next = k + 1 < nbk;
+  const unsigned tn = has_next ? unsigned(kn / 64) : ~0u;    // tile of the next pivot
+  const unsigned own = has_next ? tn * (tn + 1) / 2 + tn : 0u;  // its linear index
+  unsigned lb = blockIdx.x;
+  if (lb == 0) lb = own;
+  else if (lb == own) lb = 0;
+  const bool own_next = has_next && lb == own;
+  unsigned ti, tj;
+  gj_tile_of(lb, ti, tj);
+  const int64_t i0 = int64_t(ti) * 64, j0 = int64_t(tj) * 64;
+  double aold[4][4];  // the tile's A values, requested first
+  {
+    const int ww = threadIdx.x >> 6, ll = threadIdx.x & 63;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int64_t i = i0 + 16 * ww + (ll >> 4) + 4 * e, jj = j0 + 16 * b + (ll & 15);
+        aold[b][e] = (i < np && jj < np) ? A[i * np + jj] : 0.0;
+      }
+  }
+  // C[r][t] = s(i) R[t][i] for the tile's rows i = i0 + r (0 past np)
+  for (int e = threadIdx.x; e < 64 * GJB; e += BS) {
+    const int t = e / 64, r = e % 64;
+    const int64_t i = i0 + r;
+    const double v = i < np ? rowp[t * np + i] : 0.0;
+    cp[r][t] = i < kb ? -v : v;
+  }
+  // R rows of the tile's columns (0 inside K and past np), staged in xp
+  for (int e = threadIdx.x; e < GJB * 64; e += BS) {
+    const int t = e / 64, c = e % 64;
+    const int64_t jj = j0 + c;
+    xp[t][c] = (jj < np && (jj < kb || jj >= kb + GJB)) ? rowp[t * np + jj] : 0.0;
+  }
+  __syncthreads();
+  // X column j0 + c2, rows rb .. rb + 7 (one wave): P^-1 R[j] outside K,
+  // P^-1 itself inside K, 0 past np.  P^-1 rows are wave-uniform (scalar loads).
+  const int c2 = threadIdx.x & 63;
+  const int rb = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6) * (GJB / 4);
+  const int64_t j = j0 + c2;
+  const bool jin = j >= kb && j < kb + GJB;
+  const bool active = j < np && !jin;
   double col[GJB];
 #pragma unroll
-  for (int t = 0; t < GJB; ++t) col[t] = active ? A[(kb + t) * np + j] : 0.0;
-  __syncthreads();  // every old value read before any row-panel write below
-  if (!given) {
-    gj_invert_pivot(a);
-    if (blockIdx.x == 0)
-      for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv[e] = a[e / GJB][e % GJB];
-  }
-  if (!active) return;
-  const int rb = (threadIdx.x / GJ_COLS) * (GJB / 4);
+  for (int t = 0; t < GJB; ++t) col[t] = xp[t][c2];
+  __syncthreads();
 #pragma unroll
   for (int r = rb; r < rb + GJB / 4; ++r) {
     double sum = 0.0;
+    if (active) {
 #pragma unroll
-    for (int t = 0; t < GJB; ++t) sum += a[r][t] * col[t];
-    A[(kb + r) * np + j] = sum;
-  }
-}
-
-// Launch 2 on the FP64 matrix cores (v_mfma_f64_16x16x4f64): S = C X over
-// the 64x64 tile, wave w computing rows 16w..16w+15 as 4 blocks of 16x16,
-// GJB/4 K slices each.  Lane layouts: lane l holds A[l%16][l/16] and
-// B[l/16][l%16] of a K slice; output element i of lane l is C[l/16 + 4i][l%16].
-// The workgroup whose tile holds the next pivot block (rows and columns
-// [kb + GJB, kb + 2 GJB), inside one 64x64 tile) runs first (hardware block
-// (0, 0)), keeps that block's updated values in LDS and inverts it while the
-// other tiles are updated: the next panel launch then starts from P^-1.
-typedef double gj_d4 __attribute__((ext_vector_type(4)));
-__global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, int k, int nbk,
-                                                        const double* __restrict__ colsave,
-                                                        const double* __restrict__ pinv,
-                                                        double* __restrict__ pinv_next) {
-  // X[t][j] = new row panel A[K][j] for j outside K, P^-1[t][j - kb] inside K;
-  // S = C X gives the trailing update (A -= S) and the column panel (A = -S)
-  static_assert(BS == 256 && GJB % 4 == 0, "four waves, K slices of 4");
-  __shared__ double cp[64][GJB + 1];
-  __shared__ double xp[GJB][64 + 1];
-  __shared__ double piv[GJB][GJB + 1];
-  const int64_t kb = int64_t(k) * GJB, kn = kb + GJB;
-  const unsigned tn = k + 1 < nbk ? unsigned(kn / 64) : ~0u;  // tile of the next pivot
-  unsigned bx = blockIdx.x, by = blockIdx.y;
-  if (bx == 0 && by == 0) { bx = tn == ~0u ? 0u : tn; by = bx; }
-  else if (bx == tn && by == tn) { bx = 0; by = 0; }
-  const bool own_next = tn != ~0u && bx == tn && by == tn;
-  const int64_t i0 = int64_t(by) * 64, j0 = int64_t(bx) * 64;
-  for (int t = threadIdx.x; t < 64 * GJB; t += BS) {
-    const int r = t / GJB, c = t % GJB;
-    cp[r][c] = (i0 + r < np) ? colsave[(i0 + r) * GJB + c] : 0.0;
-    const int r2 = t / 64, c2 = t % 64;
-    const int64_t jj = j0 + c2;
-    double x = 0.0;
-    if (jj < np) x = (jj >= kb && jj < kb + GJB) ? pinv[r2 * GJB + (jj - kb)] : A[(kb + r2) * np + jj];
-    xp[r2][c2] = x;
+      for (int t = 0; t < GJB; ++t) sum += pinv[r * GJB + t] * col[t];
+    } else if (jin) {
+      sum = pinv[r * GJB + (j - kb)];
+    }
+    xp[r][c2] = sum;
   }
   __syncthreads();
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63, lr = l & 15, lk = l >> 4;
@@ -1113,33 +1174,53 @@ __global__ __launch_bounds__(BS) void gj_update2_kernel(double* A, int64_t np, i
     for (int b = 0; b < 4; ++b)
       acc[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xp[4 * sl + lk][16 * b + lr], acc[b], 0, 0, 0);
   }
+  // next row panel: rows of the next block in its tile row (direct), its
+  // columns in the tiles below its tile (transposed)
+  const bool below = has_next && ti > tn;
 #pragma unroll
   for (int b = 0; b < 4; ++b) {
-    const int64_t j = j0 + 16 * b + lr;
-    if (j >= np) continue;
-    const bool jcol = j >= kb && j < kb + GJB;
+    const int64_t jj = j0 + 16 * b + lr;
+    if (jj >= np) continue;
+    const bool jcol = jj >= kb && jj < kb + GJB;
+    const bool jnext = has_next && jj >= kn && jj < kn + GJB;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const int64_t i = i0 + 16 * w + lk + 4 * e;
       if (i >= np) continue;
-      const bool irow = i >= kb && i < kb + GJB;
-      if (irow) {
-        if (jcol) A[i * np + j] = pinv[(i - kb) * GJB + (j - kb)];
-        // rows in K outside the pivot columns: the new row panel, already written
-      } else if (jcol) {
-        A[i * np + j] = -acc[b][e];
-      } else {
-        const double v = A[i * np + j] - acc[b][e];
-        A[i * np + j] = v;
-        if (own_next && i >= kn && i < kn + GJB && j >= kn && j < kn + GJB)
-          piv[i - kn][j - kn] = v;
-      }
+      double v;
+      if (i >= kb && i < kb + GJB) v = xp[i - kb][16 * b + lr];  // row panel, P^-1 inside K
+      else if (jcol) v = -acc[b][e];                              // column panel
+      else v = aold[b][e] - acc[b][e];                            // trailing update
+      A[i * np + jj] = v;
+      const bool inext = has_next && i >= kn && i < kn + GJB;
+      if (inext) rowp_next[(i - kn) * np + jj] = v;
+      if (below && jnext) rowp_next[(jj - kn) * np + i] = v;
+      if (own_next && inext && jnext) pb[i - kn][jj - kn] = v;
     }
   }
   if (!own_next) return;
   __syncthreads();
-  gj_invert_pivot(piv);
-  for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv_next[e] = piv[e / GJB][e % GJB];
+  gj_invert_pivot(pb, pub);
+  for (int e = threadIdx.x; e < GJB * GJB; e += BS) pinv_next[e] = pb[e / GJB][e % GJB];
+}
+
+// upper tile (I, J), I < J, of linear index b = J (J - 1) / 2 + I: the
+// transpose of lower tile (J, I)
+__global__ __launch_bounds__(BS) void gj_mirror_kernel(double* A, int64_t np) {
+  __shared__ double tl[64][64 + 1];
+  unsigned J, I;
+  gj_tile_of(blockIdx.x, J, I);  // lower index pair (J', I) with I <= J'
+  ++J;                           // strictly lower: J' = J - 1 enumerates J >= I + 1
+  const int64_t i0 = int64_t(I) * 64, j0 = int64_t(J) * 64;
+  for (int e = threadIdx.x; e < 64 * 64; e += BS) {
+    const int r = e / 64, c = e % 64;  // row j0 + r, column i0 + c of the lower tile
+    if (j0 + r < np && i0 + c < np) tl[r][c] = A[(j0 + r) * np + i0 + c];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < 64 * 64; e += BS) {
+    const int r = e / 64, c = e % 64;  // row i0 + r, column j0 + c of the upper tile
+    if (i0 + r < np && j0 + c < np) A[(i0 + r) * np + j0 + c] = tl[c][r];
+  }
 }
 
 unsigned gsz(int64_t work) {
@@ -1643,15 +1724,19 @@ extern "C" int tl_setup_mg(tl_problem* P, const tl_mg_opts* o) {
     }
     const int nbk = int(M.cnp / GJB);
     const unsigned nt = unsigned((M.cnp + 63) / 64);
-    TL_TRY(M.gjw.alloc(sizeof(double) * (size_t(M.cnp) * GJB + 2 * GJB * GJB)));
-    double* colsave = M.gjw.as<double>();
-    double* pinv[2] = {colsave + size_t(M.cnp) * GJB, colsave + size_t(M.cnp) * GJB + GJB * GJB};
+    const size_t panel = size_t(M.cnp) * GJB;
+    TL_TRY(M.gjw.alloc(sizeof(double) * (2 * panel + 2 * GJB * GJB)));
+    double* w = M.gjw.as<double>();
+    double* rowp[2] = {w, w + panel};
+    double* pinv[2] = {w + 2 * panel, w + 2 * panel + GJB * GJB};
+    hipLaunchKernelGGL(gj_init_kernel, dim3(nt), dim3(BS), 0, s, A, M.cnp, rowp[0], pinv[0]);
     for (int k = 0; k < nbk; ++k) {
-      hipLaunchKernelGGL(gj_panel_kernel, dim3(unsigned((M.cnp + GJ_COLS - 1) / GJ_COLS)), dim3(BS), 0, s, A,
-                         M.cnp, k, colsave, pinv[k & 1], int(k > 0));
-      hipLaunchKernelGGL(gj_update2_kernel, dim3(nt, nt), dim3(BS), 0, s, A, M.cnp, k, nbk, colsave,
-                         pinv[k & 1], pinv[(k + 1) & 1]);
+      const int c = k & 1, n = c ^ 1;
+      hipLaunchKernelGGL(gj_step_kernel, dim3(nt * (nt + 1) / 2), dim3(BS), 0, s, A, M.cnp, k, nbk,
+                         rowp[c], pinv[c], rowp[n], pinv[n]);
     }
+    if (nt > 1)
+      hipLaunchKernelGGL(gj_mirror_kernel, dim3(nt * (nt - 1) / 2), dim3(BS), 0, s, A, M.cnp);
     TL_CHECK_LAUNCH();
   }
   M.ready = true;
