@@ -101,6 +101,53 @@ def cpu_baseline(sample=(96, 48, 48), nlevels=4):
             "nlevels": nlevels}
 
 
+def cpu_full(nelx, nely, nelz, nlevels):
+    """One-off measurement of the CPU path at full size (not part of the
+    default run, which times the bounded 96x48x48 sample): the oracle's FP64
+    GMG-PCG (scipy CSR operators, Galerkin hierarchy, direct coarsest solve
+    per V-cycle) on the headline cantilever and density, to relres 1e-8, on
+    one core.  Progress lines come from the phases and every 10th V-cycle.
+    Prints one JSON line with the phase times, iterations and the peak
+    resident memory of the process."""
+    import resource
+    from oracle import fem, mg
+    from topoptlab_amd.workloads import gaussian_random_density
+    rss = lambda: resource.getrusage(resource.RUSAGE_SELF).ru_maxrss / 2**20  # GiB
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    x = gaussian_random_density(nelx, nely, nelz, seed=0)
+    f, fixed, free = fem.clamped_lineload_3d(nelx, nely, nelz, ndof)
+    KE = fem.ke_elast_3d()
+    edof = fem.edofmat_3d(nelx, nely, nelz, 3)
+    t0 = time.perf_counter()
+    K = fem.apply_bc(fem.assemble_K(KE, edof, fem.simp(x), ndof), free).tocsr()
+    t1 = time.perf_counter()
+    log(f"cpu full: assembly {t1 - t0:.1f} s, {K.nnz} nonzeros, peak RSS {rss():.1f} GiB")
+    h = mg.Hierarchy(K, nelx, nely, nelz, 3, fixed, nlevels)
+    t2 = time.perf_counter()
+    log(f"cpu full: Galerkin hierarchy ({nlevels} levels, coarsest {h.A[-1].shape[0]} DOF) "
+        f"{t2 - t1:.1f} s, peak RSS {rss():.1f} GiB")
+    ncyc = [0]
+
+    def vcycle(r):
+        ncyc[0] += 1
+        if ncyc[0] % 10 == 0:
+            log(f"cpu full: {ncyc[0]} V-cycles, {time.perf_counter() - t2:.1f} s of PCG")
+        return h.vcycle(r)
+
+    b = f[free, 0]
+    u, info, it = mg.pcg(K, b, vcycle, rtol=1e-8)
+    t3 = time.perf_counter()
+    relres = float(np.linalg.norm(K @ u - b) / np.linalg.norm(b))
+    out = {"cpu_full": f"{nelx}x{nely}x{nelz}", "dof": ndof, "levels": nlevels,
+           "s_assembly": round(t1 - t0, 2), "s_hierarchy": round(t2 - t1, 2),
+           "s_pcg": round(t3 - t2, 2), "s_total": round(t3 - t0, 2), "pcg_iterations": it,
+           "converged": info == 0, "relres_true": relres, "compliance": float(b @ u),
+           "peak_rss_GiB": round(rss(), 2), "cores": 1}
+    log(f"cpu full: PCG {it} iterations {t3 - t2:.1f} s, total {t3 - t0:.1f} s, "
+        f"peak RSS {rss():.1f} GiB")
+    print(json.dumps(out), flush=True)
+
+
 def device_vs_oracle(dev, base, steps=3):
     """The CPU-baseline sample solved on the device with the same hierarchy
     depth: ms per solve at that size, and the device's compliance and
@@ -515,6 +562,9 @@ def main():
                     help="with --batched: time only the lockstep batch solve")
     ap.add_argument("--calibrate-read", action="store_true",
                     help="time a 235 MB device read after 1 GB written, after 1 GB read and warm")
+    ap.add_argument("--cpu-full", action="store_true",
+                    help="time the oracle's FP64 GMG-PCG at the full grid on the host "
+                         "(one-off measurement, minutes; no GPU use)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 256 MiB (5 launches) so that "
                          "FETCH_SIZE/WRITE_SIZE counters can be scaled to bytes")
@@ -540,6 +590,9 @@ def main():
                 best = min(best, ev[0].elapsed_time(ev[1]))
             log(f"read calibration, {name}: {best * 1e3:.1f} us, "
                 f"{a.numel() * 8 / (best * 1e-3) / 1e12:.2f} TB/s for {a.numel() * 8 / 1e6:.1f} MB")
+        return
+    if args.cpu_full:
+        cpu_full(args.nelx, args.nelx // 2, args.nelx // 2, args.levels)
         return
     if args.calibrate_copy:
         import torch
