@@ -178,6 +178,44 @@ def test_pcg_gmg_fp64_large_level_elasticity_fixture(gpu):
     op.close()
 
 
+def test_c2_solve_matches_oracle_fixture(gpu):
+    """The headline workload at full size (192x96x96 H8 cantilever, 5.45M
+    DOF, x = 0 face clamped, unit line load, Gaussian random field density
+    seed 0; FP64 V(1,1) over 5 levels, relres 1e-8) against the oracle's FP64
+    GMG-PCG on the same problem, committed by
+    tests/golden/make_c2_oracle_fixture.py (about 75 s and 47 GiB of host
+    memory): the same iteration count (+-1), the compliance f.u within 1e-8
+    relative (the bar for C2), ||u|| within 1e-8 and u at 4096 seeded free
+    DOFs within 1e-6."""
+    import hashlib
+    import torch
+    from tests.conftest import GOLDEN
+    from topoptlab_amd.fem import StiffnessOperator
+    fx = np.load(GOLDEN / "c2_oracle_192x96x96.npz")
+    nelx, nely, nelz = (int(v) for v in fx["dims"])
+    say("c2 oracle fixture", (nelx, nely, nelz))
+    ndof = 3 * (nelx + 1) * (nely + 1) * (nelz + 1)
+    x = workloads.gaussian_random_density(nelx, nely, nelz, seed=0)
+    assert hashlib.sha256(x.tobytes()).hexdigest() == str(fx["density_sha256"])
+    f, fixed, free = fem.clamped_lineload_3d(nelx, nely, nelz, ndof)
+    op = StiffnessOperator(nelx, nely, nelz, KE=fem.ke_elast_3d(), fixed=fixed, device=gpu)
+    op.set_density(x)
+    op.setup_mg(nlevels=int(fx["nlevels"]), precision="fp64")
+    ft = torch.as_tensor(f[:, 0], device=gpu)
+    u, inf = op.solve(ft, rtol=1e-8)
+    c = float(torch.dot(ft, u).item())
+    uf = u[op._free_t].cpu().numpy()
+    c_ref = float(fx["compliance"])
+    say("iterations", inf["iterations"], "oracle", int(fx["iterations"]),
+        "compliance rel diff", abs(c - c_ref) / c_ref)
+    assert abs(inf["iterations"] - int(fx["iterations"])) <= 1
+    assert abs(c - c_ref) <= 1e-8 * abs(c_ref)
+    assert abs(np.linalg.norm(uf) - float(fx["unorm"])) <= 1e-8 * float(fx["unorm"])
+    assert rel(uf[fx["idx_free"]], fx["u_idx"]) < 1e-6
+    assert inf["relres_true"] < 1e-8
+    op.close()
+
+
 @pytest.mark.parametrize("gamma_from", [1, 2])
 def test_pcg_wcycle_fp64_matches_oracle(gpu, gamma_from):
     """FP64 W-cycle (gamma = 2 from level gamma_from): same iteration count
