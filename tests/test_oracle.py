@@ -124,6 +124,23 @@ def test_bcs_match_package():
     assert list(fixed) == [18, 19, 20, 21, 22]
 
 
+
+def test_rectangle_2d_heat_bc():
+    """example_bc/heat_conduction.py:106-145 stated directly: the x = 0 edge
+    (nodes 0..nely, one DOF each) fixed, a unit source on the x = L edge,
+    zero initial state, no springs."""
+    nelx, nely = 7, 5
+    ndof = (nelx + 1) * (nely + 1)
+    u, f, fixed, free, springs = example_bc.rectangle_2d(nelx, nely, ndof)
+    assert springs is None and u.shape == (ndof, 1) and not u.any()
+    assert np.array_equal(fixed, np.arange(nely + 1))
+    assert np.array_equal(free, np.arange(nely + 1, ndof))
+    src = np.flatnonzero(f[:, 0])
+    assert np.array_equal(src, np.arange(nelx * (nely + 1), ndof)) and np.all(f[src, 0] == 1.0)
+    # the source edge is the last column of nodes: ix = nelx for every iy
+    assert np.all(src // (nely + 1) == nelx)
+
+
 @pytest.mark.parametrize("nelx,nely,nelz", [(4, 3, 2), (5, 2, 6)])
 @pytest.mark.parametrize("force_mode", ["linear", "constant"])
 def test_cantilever_3d_closed_form(nelx, nely, nelz, force_mode):

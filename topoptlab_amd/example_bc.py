@@ -6,6 +6,7 @@
 * ``cantilever_2d_twoloads`` <- topoptlab/example_bc/lin_elast.py:141-182
 * ``cantilever_3d`` <- topoptlab/example_bc/lin_elast.py:274-338
 * ``heatplate_2d``  <- topoptlab/example_bc/heat_conduction.py:5-49
+* ``rectangle_2d``  <- topoptlab/example_bc/heat_conduction.py:106-145
 * ``clamped_lineload_3d``: the 192x96x96 cantilever benchmark (not a
   reference function): x=0 face clamped, unit total downward line load on the
   bottom edge (iy = nely) of the x = L face, consistent nodal weights.
@@ -63,6 +64,15 @@ def heatplate_2d(nelx, nely, ndof, **kwargs):
     f = np.full((ndof, 1), 0.01)
     fixed = np.arange(int(nely / 2 - nely / 20), int(nely / 2 + 1 + nely / 20))
     return _out(ndof, f, fixed)
+
+
+def rectangle_2d(nelx, nely, ndof, **kwargs):
+    """Heat conduction through a rectangular beam: the nodes of the x = 0
+    edge held at zero (the heat sink), a unit source on every node of the
+    x = L edge (the last nely+1 DOFs in the node-major numbering)."""
+    f = np.zeros((ndof, 1))
+    f[ndof - (nely + 1):, 0] = 1.0
+    return _out(ndof, f, np.arange(nely + 1))
 
 
 def clamped_lineload_3d(nelx, nely, nelz, ndof, **kwargs):
