@@ -81,6 +81,32 @@ def test_ku_matches_assembled(gpu, dims, ndpn):
     op.close()
 
 
+@pytest.mark.parametrize("dims,ndpn", [((12, 8, None), 2), ((6, 4, 5), 3), ((10, 6, None), 1)])
+def test_ku_sheared_elements_generic_path(gpu, dims, ndpn):
+    """Element matrices of non-unit, sheared elements (elements.py with l and
+    g) do not match the unit-cell value pattern, so the device takes its
+    generic element-matrix path: K.u against the assembled matrix."""
+    import torch
+    from topoptlab_amd.fem import StiffnessOperator
+    say("ku sheared", dims, ndpn)
+    nelx, nely, nelz = dims
+    if nelz is None:
+        KE = (elements.lk_linear_elast_2d(l=np.array([1.5, 0.75]), g=np.array([0.3])) if ndpn == 2
+              else elements.lk_poisson_2d(l=np.array([1.5, 0.75]), g=np.array([0.3])))
+        edof = fem.edofmat_2d(nelx, nely, ndpn)
+    else:
+        KE = elements.lk_linear_elast_3d(l=np.array([1.2, 0.9, 0.6]), g=np.array([0.2, -0.1]))
+        edof = fem.edofmat_3d(nelx, nely, nelz, ndpn)
+    _, _, ndof, fixed, free, x = _problem(dims, ndpn)
+    K = fem.apply_bc(fem.assemble_K(KE, edof, fem.simp(x), ndof), free).tocsr()
+    op = StiffnessOperator(nelx, nely, nelz, KE=KE, fixed=fixed, device=gpu)
+    op.set_density(x)
+    u = np.random.default_rng(2).standard_normal(len(free))
+    y = (op @ torch.as_tensor(u, device=gpu)).cpu().numpy()
+    assert rel(y, K @ u) < 1e-13
+    op.close()
+
+
 @pytest.mark.parametrize("dims,ndpn,nlevels", [((16, 8, 8), 3, 3), ((32, 16, None), 2, 4),
                                                ((16, 16, None), 1, 3), ((8, 4, 4), 3, 2)])
 def test_pcg_gmg_fp64_matches_oracle(gpu, dims, ndpn, nlevels):

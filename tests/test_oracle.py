@@ -78,6 +78,105 @@ def test_ke_poisson():
     assert np.abs(K3.sum(1)).max() < 1e-15 and np.abs(K3 - K3.T).max() < 1e-15
 
 
+def _node_xy(l, g):
+    """Physical node coordinates of the Q4 parallelogram in the mesh-local
+    node order (element frame, y up): corner signs of LOCAL_OFFSETS_2D with
+    y flipped, mapped by the side lengths and angle."""
+    offs = elements.LOCAL_OFFSETS_2D.astype(float)
+    q = (2 * offs - 1) / 2
+    q[:, 1] *= -1
+    J = np.array([[l[0], l[1] * np.tan(g)], [0.0, l[1]]])
+    return q @ J.T
+
+
+def test_ke_parallelogram_reference_closed_forms():
+    """Non-unit, sheared elements against entries of the reference's closed
+    forms: linear_elasticity_2d.py:98-105 (plane stress, row 0 and entry
+    (1,1)), poisson_2d.py:93 (row 0), poisson_3d.py:84-88 (row 0, five
+    entries)."""
+    E, nu, l, g = 2.0, 0.3, np.array([1.3, 0.7]), 0.3
+    t, c2 = np.tan(g), np.cos(g) ** 2
+    K = elements.lk_linear_elast_2d(E=E, nu=nu, l=l, g=np.array([g]))
+    ref = [
+        (0, 0, E*(2*l[0]**2*nu - 2*l[0]**2 - 3*l[0]*l[1]*nu*t + 3*l[0]*l[1]*t + 2*l[1]**2*nu*t**2
+                  - 2*l[1]**2*t**2 - 4*l[1]**2)/(12*l[0]*l[1]*(nu**2 - 1))),
+        (0, 1, E*(-3*l[0] + 4*l[1]*t)/(24*l[0]*(nu - 1))),
+        (0, 2, E*(l[0]**2*nu - l[0]**2 - 2*l[1]**2*nu*t**2 + 2*l[1]**2*t**2 + 4*l[1]**2)
+         / (12*l[0]*l[1]*(nu**2 - 1))),
+        (0, 3, E*(-9*l[0]*nu + 3*l[0] - 4*l[1]*nu*t - 4*l[1]*t)/(24*l[0]*(nu**2 - 1))),
+        (1, 1, E*(-2*l[0]**2 + 3*l[0]*l[1]*t + l[1]**2*nu + l[1]**2 - 2*l[1]**2/c2)
+         / (6*l[0]*l[1]*(nu**2 - 1))),
+    ]
+    for i, j, v in ref:
+        assert abs(K[i, j] - v) < 1e-14 * np.abs(K).max(), (i, j)
+    P = elements.lk_poisson_2d(l=l, g=np.array([g]))
+    p0 = [l[0]/(3*l[1]) - t/2 + l[1]/(3*l[0]*c2), l[0]/(6*l[1]) - l[1]/(3*l[0]*c2),
+          -l[0]/(6*l[1]) + t/2 - l[1]/(6*l[0]*c2), -l[0]/(3*l[1]) + l[1]/(6*l[0]*c2)]
+    assert np.abs(P[0] - p0).max() < 1e-14
+    l3, g3 = np.array([1.3, 0.7, 0.9]), np.array([0.3, 0.2])
+    t0, t1 = np.tan(g3)
+    P3 = elements.lk_poisson_3d(l=l3, g=g3)
+    a, b, c = l3
+    p30 = [a*b/(9*c) + a*c/(9*b) - b*t1/6 - c*t0/6 + b*c*t0**2/(9*a) + b*c*t1**2/(9*a) + b*c/(9*a),
+           a*b/(18*c) + a*c/(18*b) - b*c*t0**2/(9*a) - b*c*t1**2/(9*a) - b*c/(9*a),
+           a*b/(36*c) - a*c/(18*b) + c*t0/6 - b*c*t0**2/(18*a) - b*c*t1**2/(18*a) - b*c/(18*a),
+           a*b/(18*c) - a*c/(9*b) - b*t1/12 + b*c*t0**2/(18*a) + b*c*t1**2/(18*a) + b*c/(18*a),
+           -a*b/(9*c) + a*c/(18*b) - c*t0/12 + b*c*t0**2/(18*a) + b*c*t1**2/(18*a) + b*c/(18*a)]
+    assert np.abs(P3[0, :5] - p30).max() < 1e-14
+
+
+def test_ke_parallelogram_rigid_modes_and_patch():
+    """For a sheared, non-square element: the three rigid-body motions of its
+    actual node positions are in the null space, and a linear displacement
+    field stores the energy area * eps : C : eps / 2 exactly."""
+    E, nu, l, g = 1.0, 0.3, np.array([1.7, 0.6]), -0.4
+    K = elements.lk_linear_elast_2d(E=E, nu=nu, l=l, g=np.array([g]))
+    X = _node_xy(l, g)
+    for u in (np.tile([1.0, 0.0], 4), np.tile([0.0, 1.0], 4),
+              np.column_stack([-X[:, 1], X[:, 0]]).ravel()):
+        assert np.abs(K @ u).max() < 1e-13
+    G = np.array([[0.3, -0.2], [0.5, 0.1]])  # u = G x
+    u = (X @ G.T).ravel()
+    eps = np.array([G[0, 0], G[1, 1], G[0, 1] + G[1, 0]])
+    C = E / (1 - nu ** 2) * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]])
+    assert abs(u @ K @ u - l[0] * l[1] * eps @ C @ eps) < 1e-13
+    # the mass matrix scales with the area (mass_scalar_2d.py:91-92)
+    assert np.isclose(elements.lm_mass_2d(l=l).sum(), l[0] * l[1])
+
+
+def test_ke_parallelepiped_rigid_modes_and_patch():
+    """H8 with side lengths l and angles g (y edge leaning by l1 tan g0, z edge
+    by l2 tan g1, both in x): the six rigid-body motions of the actual node
+    positions are in the null space of KE, and a linear displacement field
+    stores volume * eps : C : eps / 2 with the element's isotropic tensor."""
+    E, nu = 1.0, 0.3
+    l, g = np.array([1.4, 0.8, 1.1]), np.array([0.25, -0.35])
+    K = elements.lk_linear_elast_3d(E=E, nu=nu, l=l, g=g)
+    offs = elements.LOCAL_OFFSETS_3D.astype(float)
+    q = (2 * offs - 1) / 2
+    q[:, 1] *= -1
+    J = np.diag(l)
+    J[0, 1], J[0, 2] = l[1] * np.tan(g[0]), l[2] * np.tan(g[1])
+    X = q @ J.T
+    modes = [np.tile(np.eye(3)[i], 8) for i in range(3)]
+    for a_, b_ in ((0, 1), (1, 2), (0, 2)):
+        u = np.zeros((8, 3))
+        u[:, a_], u[:, b_] = -X[:, b_], X[:, a_]
+        modes.append(u.ravel())
+    for u in modes:
+        assert np.abs(K @ u).max() < 1e-12
+    G = np.array([[0.3, -0.2, 0.1], [0.5, 0.1, -0.3], [0.2, 0.4, -0.1]])
+    u = (X @ G.T).ravel()
+    eps = np.array([G[0, 0], G[1, 1], G[2, 2], G[1, 2] + G[2, 1], G[0, 2] + G[2, 0],
+                    G[0, 1] + G[1, 0]])
+    f = E / ((1 + nu) * (1 - 2 * nu))
+    C = np.zeros((6, 6))
+    C[:3, :3] = nu * f
+    C[np.arange(3), np.arange(3)] = (1 - nu) * f
+    C[np.arange(3, 6), np.arange(3, 6)] = (1 - nu) / 2 * f
+    assert abs(u @ K @ u - np.prod(l) * eps @ C @ eps) < 1e-12
+
+
 def test_simp_dx_numerical():
     x = np.linspace(0.05, 1, 13)
     h = 1e-6

@@ -53,17 +53,32 @@ def create_edofMat_3d(nelx, nely, nelz, nnode_dof, dtype=np.int32, **kwargs):
             n3.astype(dtype), n4.astype(dtype))
 
 
-def _check_unit(l, g, ndim):
+def _jacobian(l, g, ndim):
+    """Edge vectors of the element as the columns of J (physical = J q for
+    the unit cell coordinates q, element frame with y up): side lengths l and
+    parallelogram angles g as the reference's closed forms use them
+    (elements/linear_elasticity_2d.py:69-164, poisson_2d.py:68-97,
+    poisson_3d.py): in 2D the y edge leans by l1 tan(g0) in x; in 3D the y
+    edge by l1 tan(g0) and the z edge by l2 tan(g1), both in x."""
     lv = np.broadcast_to(np.asarray(l, dtype=float), (ndim,))
-    gv = np.asarray(g, dtype=float).ravel()
-    if not np.all(lv == 1.0) or not np.all(gv == 0.0):
-        raise NotImplementedError("only unit, unsheared elements (l=1, g=0) are supported")
+    gv = np.zeros(ndim - 1)
+    gin = np.asarray(g, dtype=float).ravel()
+    gv[:min(len(gin), ndim - 1)] = gin[:ndim - 1]
+    J = np.diag(lv).astype(float)
+    J[0, 1] = lv[1] * np.tan(gv[0])
+    if ndim == 3:
+        J[0, 2] = lv[2] * np.tan(gv[1])
+    return J
 
 
-def _gauss_grad(ndim):
-    """Shape-function gradients on the reference cell [-1/2, 1/2]^ndim at the
-    2^ndim Gauss points; returns list of (weight, dN (nnode, ndim))."""
+def _gauss_grad(ndim, l=None, g=None):
+    """Physical shape-function gradients at the 2^ndim Gauss points of the
+    element with side lengths l and angles g (unit cube by default); returns
+    list of (weight * det J, dN (nnode, ndim)).  The Jacobian is constant on
+    a parallelogram/parallelepiped, so 2-point Gauss rules stay exact."""
     offs = LOCAL_OFFSETS_2D if ndim == 2 else LOCAL_OFFSETS_3D
+    J = np.eye(ndim) if l is None else _jacobian(l, np.zeros(ndim - 1) if g is None else g, ndim)
+    Jinv, detJ = np.linalg.inv(J), float(np.linalg.det(J))
     # reference-cell corner signs: the mesh index iy grows downwards while the
     # element's eta axis points up (elements/bilinear_quadrilateral.py:209-212)
     sgn = 2 * offs - 1
@@ -78,20 +93,19 @@ def _gauss_grad(ndim):
         for a in range(ndim):
             others = np.prod(np.delete(fac, a, axis=1), axis=1)
             dN[:, a] = sgn[:, a] * others
-        out.append((1.0 / 2 ** ndim, dN))
+        out.append((detJ / 2 ** ndim, dN @ Jinv))
     return out
 
 
 def lk_linear_elast_2d(E=1.0, nu=0.3, plane_stress=True, l=np.array([1.0, 1.0]),
                        g=np.array([0.0]), t=1.0, **kwargs):
-    _check_unit(l, g, 2)
     if plane_stress:
         C = E / (1 - nu ** 2) * np.array([[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]])
     else:
         C = E / ((1 + nu) * (1 - 2 * nu)) * np.array([[1 - nu, nu, 0], [nu, 1 - nu, 0],
                                                        [0, 0, (1 - 2 * nu) / 2]])
     K = np.zeros((8, 8))
-    for w, dN in _gauss_grad(2):
+    for w, dN in _gauss_grad(2, l, g):
         B = np.zeros((3, 8))
         B[0, 0::2] = dN[:, 0]
         B[1, 1::2] = dN[:, 1]
@@ -106,14 +120,13 @@ def lk_linear_elast_3d(E=1.0, nu=0.3, l=np.array([1.0, 1.0, 1.0]), g=np.array([0
     """H8 stiffness with the reference's isotropic tensor
     (topoptlab/stiffness_tensors.py:179-204), whose shear diagonal is
     E(1-nu)/(2(1+nu)(1-2nu)); the reference's closed form uses the same."""
-    _check_unit(l, g, 3)
     f = E / ((1 + nu) * (1 - 2 * nu))
     C = np.zeros((6, 6))
     C[:3, :3] = nu * f
     C[np.arange(3), np.arange(3)] = (1 - nu) * f
     C[np.arange(3, 6), np.arange(3, 6)] = (1 - nu) / 2 * f
     K = np.zeros((24, 24))
-    for w, dN in _gauss_grad(3):
+    for w, dN in _gauss_grad(3, l, g):
         B = np.zeros((6, 24))
         for a in range(3):
             B[a, a::3] = dN[:, a]
@@ -125,17 +138,17 @@ def lk_linear_elast_3d(E=1.0, nu=0.3, l=np.array([1.0, 1.0, 1.0]), g=np.array([0
 
 
 def lk_poisson_2d(k=1.0, l=np.array([1.0, 1.0]), g=np.array([0.0]), t=1.0, **kwargs):
-    _check_unit(l, g, 2)
     K = np.zeros((4, 4))
-    for w, dN in _gauss_grad(2):
+    for w, dN in _gauss_grad(2, l, g):
         K += w * dN @ dN.T
     return t * k * K
 
 
 def lm_mass_2d(p=1.0, l=np.array([1.0, 1.0]), t=1.0, **kwargs):
     """Q4 scalar mass matrix p * int N N^T (elements/mass_scalar_2d.py:68-95),
-    by 2x2 Gauss quadrature (exact for the biquadratic products)."""
-    _check_unit(l, np.array([0.0]), 2)
+    by 2x2 Gauss quadrature (exact for the biquadratic products), scaled by
+    the element area l0*l1 as the reference does."""
+    lv = np.broadcast_to(np.asarray(l, dtype=float), (2,))
     offs = LOCAL_OFFSETS_2D
     sgn = 2 * offs - 1
     sgn[:, 1] *= -1
@@ -144,7 +157,7 @@ def lm_mass_2d(p=1.0, l=np.array([1.0, 1.0]), t=1.0, **kwargs):
     for q in itertools.product((-gp, gp), repeat=2):
         N = np.prod(0.5 + sgn * np.array(q)[None, :], axis=1)
         M += 0.25 * np.outer(N, N)
-    return p * t * M
+    return p * t * lv[0] * lv[1] * M
 
 
 def lk_screened_poisson_2d(k=1.0, l=np.array([1.0, 1.0]), g=np.array([0.0]), t=1.0, **kwargs):
@@ -155,8 +168,7 @@ def lk_screened_poisson_2d(k=1.0, l=np.array([1.0, 1.0]), g=np.array([0.0]), t=1
 
 
 def lk_poisson_3d(k=1.0, l=np.array([1.0, 1.0, 1.0]), g=np.array([0.0, 0.0]), **kwargs):
-    _check_unit(l, g, 3)
     K = np.zeros((8, 8))
-    for w, dN in _gauss_grad(3):
+    for w, dN in _gauss_grad(3, l, g):
         K += w * dN @ dN.T
     return k * K
