@@ -318,6 +318,7 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True, lockstep_
     ms_step = timed(lambda: (setup_all(), solve_all()), steps)
     ms_setup = timed(setup_all, steps)
     ms_solve = timed(solve_all, steps)  # lockstep: all systems in every launch
+    ms_lock_eager = timed(lambda: solve_all(graphs=False), steps)  # same, chunks not captured
     ms_streams = ms_eager = ms_one_stream = ms_seq = None
     if not lockstep_only:  # the other ways to run the same systems
         solve_all(lockstep=False)  # captures every instance's chunk graph for this hierarchy
@@ -334,7 +335,7 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True, lockstep_
     fmt = lambda v: "-" if v is None else f"{v:.1f}"
     r2 = lambda v: None if v is None else round(v, 2)
     log(f"batched: step {ms_step:.1f} ms (setup {ms_setup:.1f}, lockstep batch solve "
-        f"{ms_solve:.1f}, per-instance graphs on {nstreams} streams {fmt(ms_streams)}, eager "
+        f"{ms_solve:.1f} (eager {ms_lock_eager:.1f}), per-instance graphs on {nstreams} streams {fmt(ms_streams)}, eager "
         f"{fmt(ms_eager)}, one stream {fmt(ms_one_stream)}, one at a time {fmt(ms_seq)}); "
         f"iterations {min(its)}..{max(its)}, max true relres {max(rr):.2e}")
     out = {
@@ -349,7 +350,8 @@ def bench_batched(dev, steps=2, nstreams=16, chunk=8, cpu_sample=True, lockstep_
                                    * (sp["drawn"][1] + 1) for sp in specs)),
         "dof_rhs_total": int(dof_rhs),
         "ms_per_step": round(ms_step, 2), "ms_setup": round(ms_setup, 2),
-        "ms_batch_solve": round(ms_solve, 2), "batch_solve_mode": "lockstep",
+        "ms_batch_solve": round(ms_solve, 2), "batch_solve_mode": "lockstep, chunk graphs",
+        "ms_batch_solve_lockstep_eager": round(ms_lock_eager, 2),
         "ms_batch_solve_stream_graphs": r2(ms_streams),
         "ms_batch_solve_eager": r2(ms_eager),
         "ms_batch_solve_one_stream": r2(ms_one_stream),

@@ -109,7 +109,8 @@ int tl_solve(tl_problem* p, const double* f, double* u,
 typedef struct {
   int32_t nstreams;    /* concurrent HIP streams (<= 0: 4) */
   int32_t chunk;       /* PCG iterations per host check (rounded up to even; <= 0: 8) */
-  int32_t use_graphs;  /* 1: each instance replays its chunk as a captured hipGraph */
+  int32_t use_graphs;  /* 1: the chunk of PCG iterations between host checks is captured as a
+                          hipGraph and replayed (per instance, or the whole lockstep chunk) */
   int32_t lockstep;    /* 1: all systems iterate together, one launch per kernel and place
                           in the cycle covering every system (2D handles with GMG V(1,1);
                           otherwise the per-instance streams are used) */

@@ -130,11 +130,12 @@ def test_batch_matches_single_solves(gpu):
     say("batch set-up on 3 streams")
     batch_setup(ops, [torch.as_tensor(x, device=gpu) for x in xs], lvs, precision="fp32",
                 nstreams=3)
-    # lockstep (one launch per kernel and place in the cycle for all systems),
-    # then per-instance streams with and without captured graphs; each twice
-    # in a row (the second call reuses what the first set up)
-    for lockstep, graphs in ((True, True), (True, True), (False, True), (False, False),
-                             (False, True)):
+    # lockstep (one launch per kernel and place in the cycle for all systems;
+    # chunks captured as graphs and replayed, and eager), then per-instance
+    # streams with and without captured graphs; the graph forms twice in a row
+    # (the second call reuses what the first set up)
+    for lockstep, graphs in ((True, True), (True, True), (True, False), (False, True),
+                             (False, False), (False, True)):
         say("batch solve, lockstep", lockstep, "graphs", graphs)
         U, infos = batch_solve(ops, fs, rtol=1e-8, nstreams=3, chunk=4, use_graphs=graphs,
                                lockstep=lockstep)

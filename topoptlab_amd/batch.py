@@ -13,7 +13,9 @@ of uneven grid sizes is handed to ``tl_batch_solve`` at once:
   iteration of every system is recorded once, the recordings are merged, and
   each iteration is one launch per kernel and place in the cycle covering the
   blocks of every system (a tile table maps blocks to systems); converged
-  systems drop out of the launches;
+  systems drop out of the launches; with ``use_graphs`` each chunk of
+  iterations is captured once per schedule (the set of systems still
+  iterating) as a hipGraph on a stream of its own and replayed;
 * otherwise instances run concurrently on several HIP streams; each
   instance's chunk of PCG iterations is captured once into a hipGraph and
   replayed until the device-side state reports convergence, and the load

@@ -2061,7 +2061,8 @@ bool lockstep_eligible(int n, tl_problem* const* hs, int precond) {
 }
 
 int batch_lockstep(int32_t n, tl_problem* const* hs, const double* const* f, double* const* u,
-                   const int32_t* nrhs, const tl_solve_opts* o, int chunk, tl_solve_info* infos) {
+                   const int32_t* nrhs, const tl_solve_opts* o, int chunk, bool graphs,
+                   tl_solve_info* infos) {
   // systems, their buffer layout in one arena
   struct Sys { int inst, rhs; SysBufs buf; PcgCtx ctx; Recorder rec[2]; };
   std::vector<Sys> sys;
@@ -2091,7 +2092,20 @@ int batch_lockstep(int32_t n, tl_problem* const* hs, const double* const* f, dou
       for (size_t k = 0; k < sy.buf.lv.size(); ++k) carve(sy.buf.lv[k], sz[kSysBufs + k]);
     }
   }
-  hipStream_t s = hs[0]->stream;
+  // the batch runs on a stream of its own (a graph cannot be captured on the
+  // null stream), after the work already queued on the first handle's
+  // stream; it is synchronised before this function returns
+  hipStream_t s = nullptr;
+  TL_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard { hipStream_t s; ~StreamGuard() { (void)hipStreamDestroy(s); } } sguard0{s};
+  {
+    hipEvent_t e = nullptr;
+    TL_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    const hipError_t er = hipEventRecord(e, hs[0]->stream);
+    const hipError_t ew = er == hipSuccess ? hipStreamWaitEvent(s, e, 0) : er;
+    (void)hipEventDestroy(e);
+    TL_HIP(ew);
+  }
   // initialise every system and record its two iteration parities
   for (Sys& sy : sys) {
     tl_problem* P = hs[sy.inst];
@@ -2149,15 +2163,43 @@ int batch_lockstep(int32_t n, tl_problem* const* hs, const double* const* f, dou
     return TL_OK;
   };
   TL_TRY(build());
-  int nactive = ns, built_for = ns;
-  int64_t k = 0;
-  for (;;) {
-    for (int c = 0; c < chunk; ++c, ++k) {
-      const int par = int(k & 1);
+  // one chunk of lockstep iterations; chunks start on an even iteration, so
+  // every chunk replays the same launches until the schedule is rebuilt:
+  // with graphs it is captured once per schedule and replayed
+  auto launch_chunk = [&]() -> int {
+    for (int c = 0; c < chunk; ++c) {
+      const int par = c & 1;
       for (const MergedLaunch& m : sched[par])
         TL_TRY(m.fn(s, dev_blob[par] + m.desc_off,
                     reinterpret_cast<const uint2*>(dev_blob[par] + m.tile_off), m.nblocks, m.lds));
     }
+    return TL_OK;
+  };
+  hipGraphExec_t gexec = nullptr;
+  struct ExecGuard { hipGraphExec_t& g; ~ExecGuard() { if (g) (void)hipGraphExecDestroy(g); } } eg{gexec};
+  int nactive = ns, built_for = ns;
+  int64_t k = 0;
+  for (;;) {
+    if (graphs) {
+      if (!gexec) {
+        TL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_chunk();
+        hipGraph_t g = nullptr;
+        const hipError_t ec = hipStreamEndCapture(s, &g);
+        if (rc != TL_OK) {
+          if (g) (void)hipGraphDestroy(g);
+          return rc;
+        }
+        TL_HIP(ec);
+        const hipError_t ei = hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        TL_HIP(ei);
+      }
+      TL_HIP(hipGraphLaunch(gexec, s));
+    } else {
+      TL_TRY(launch_chunk());
+    }
+    k += chunk;
     hipLaunchKernelGGL(gather_done_kernel, dim3(unsigned((ns + 255) / 256)), dim3(256), 0, s,
                        stp.as<PcgState*>(), ns, dflags.as<int>());
     TL_CHECK_LAUNCH();
@@ -2173,6 +2215,10 @@ int batch_lockstep(int32_t n, tl_problem* const* hs, const double* const* f, dou
     if (nactive <= built_for * 3 / 4) {
       TL_TRY(build());
       built_for = nactive;
+      if (gexec) {
+        (void)hipGraphExecDestroy(gexec);
+        gexec = nullptr;
+      }
     }
   }
   // true residuals and results
@@ -2249,7 +2295,7 @@ int tl_batch_solve(int32_t n, tl_problem* const* hs, const double* const* f, dou
       }
     std::vector<bool> prof(n);
     for (int i = 0; i < n; ++i) { prof[i] = hs[i]->prof_on; hs[i]->prof_on = false; }
-    const int rc = batch_lockstep(n, hs, f, u, nrhs, o, chunk, infos);
+    const int rc = batch_lockstep(n, hs, f, u, nrhs, o, chunk, graphs, infos);
     for (int i = 0; i < n; ++i) hs[i]->prof_on = prof[i];
     // later work on the other handles' streams follows the batch
     for (int i = 1; i < n; ++i)
