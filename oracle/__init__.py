@@ -30,7 +30,9 @@ Pinning status (what checks each part independently of this restatement):
   (exact reproduction of coarse multilinear fields, injection at coincident
   nodes, row sums 1, SPD coarse operators) and convergence to the direct
   solution.  The reference's own ``gmg.create_interpolator`` is never called
-  by the reference and is index-inconsistent for non-square grids, so its
-  output is *not* reproduced: parity unpinned against the reference for P
-  itself, pinned by the invariants above.
+  by the reference and raises ValueError on every grid (its corner columns
+  index past the coarse grid; recorded in tests/golden/interpolator_vectors.npz
+  from the reference checkout), so there is no reference P to reproduce; P is
+  the intended shape-function prolongation, pinned by the invariants above
+  and by the reference's shape-function weights at its relative coordinates.
 """

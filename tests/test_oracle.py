@@ -542,3 +542,33 @@ def test_passive_golden(kats):
     # without the flags the loop converges elsewhere: the flags matter
     _, obj_free = topopt.topopt_loop(nelx, nely, volfrac, rmin, ft=ft, bcs=fem.cantilever_2d)
     assert abs(obj_free - ref) > 1.0
+
+
+def test_reference_interpolator_raises_on_every_grid():
+    """gmg.create_interpolator (gmg.py:34-146) as the reference ships it:
+    the fixture written from the reference checkout records a ValueError on
+    every grid, square or not, 2D and 3D, because the corner columns of the
+    last node row and column (gmg.py:117-121) index past the coarse grid.  So
+    the reference has no P to reproduce; the oracle's prolongator is the
+    intended operator (shape functions of bilinear_quadrilateral.py:209-212 /
+    trilinear_hexahedron.py at stride 2), pinned by test_prolongator_invariants
+    and here by the weights the reference's shape-function evaluation gives at
+    its relative coordinates (gmg.py:130-143)."""
+    import json
+    d = np.load(GOLDEN / "interpolator_vectors.npz")
+    meta = json.loads(str(d["meta"]))
+    assert len(meta) >= 10
+    for e in meta:
+        assert e.get("error", "").startswith("ValueError: axis 1 index"), e
+    # the weights the reference assigns at stride 2: fine node (x, y) has
+    # xi = x%2 - 1 and eta = 1 - y%2 in its coarse cell; the 2D shape
+    # functions there give 1 at coincident nodes, 1/2 on cell edges, 1/4 in
+    # cell centres, each on the coarse nodes that the oracle's P uses
+    N = lambda xi, eta: 0.25 * np.array([(1 - xi) * (1 - eta), (1 + xi) * (1 - eta),
+                                         (1 + xi) * (1 + eta), (1 - xi) * (1 + eta)])
+    P = mg.prolongator(4, 4, None, 1).toarray()
+    for x in range(5):
+        for y in range(5):
+            w = N(x % 2 - 1.0, 1.0 - y % 2)
+            got = np.sort(P[x * 5 + y][P[x * 5 + y] != 0])
+            assert np.allclose(got, np.sort(w[w != 0])), (x, y)
