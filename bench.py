@@ -242,7 +242,10 @@ def kernel_kinds(dims, vcycle, nlevels, fp64_mfma=False):
     mfma = vcycle in ("bf16", "fp16")
     # compute-bound kinds carry bound "mfma" (the compute roof) with the
     # peak of the unit that runs them: FP64 vector FMAs and FP64 MFMA share
-    # 78.6 TFLOP/s on MI355X, FP32 vector FMAs 157.3
+    # 78.6 TFLOP/s on MI355X, FP32 vector FMAs 157.3 (a figure that counts
+    # packed FP32; the library is built without packed FP32 instructions, so
+    # half of it is the ceiling these kernels can reach and the fraction
+    # reported against 157.3 is the conservative one)
     if mfma:
         vk = f"kucol_kernel {vcycle.upper()}-input MFMA (fp32 accumulate)"
         vbound, vpeak = "hbm", HBM_PEAK_GBS
