@@ -54,9 +54,13 @@ def _problem(dims, ndpn, seed=0, bc="cantilever"):
 # that each plane copy takes several rounds (4x300x3: ny = 301 > 256)
 CASES = [((12, 8, None), 2), ((10, 6, None), 1), ((6, 4, 5), 3), ((5, 4, 3), 1),
          ((24, 16, 20), 3), ((300, 7, None), 2), ((4, 300, 3), 3), ((40, 30, 6), 1)]
+# one element, and grids one element thick (the plane walk with two node
+# planes, node lines of two nodes)
+TINY = [((1, 1, None), 2), ((5, 1, None), 1), ((1, 4, None), 2), ((1, 1, 1), 3),
+        ((3, 2, 1), 3), ((1, 3, 2), 1), ((2, 1, 2), 3), ((1, 1, 3), 1)]
 
 
-@pytest.mark.parametrize("dims,ndpn", CASES)
+@pytest.mark.parametrize("dims,ndpn", CASES + TINY)
 def test_ku_matches_assembled(gpu, dims, ndpn):
     import torch
     from topoptlab_amd.fem import StiffnessOperator
@@ -107,12 +111,18 @@ def test_ku_sheared_elements_generic_path(gpu, dims, ndpn):
     op.close()
 
 
-@pytest.mark.parametrize("dims,ndpn,nlevels", [((16, 8, 8), 3, 3), ((32, 16, None), 2, 4),
-                                               ((16, 16, None), 1, 3), ((8, 4, 4), 3, 2)])
-def test_pcg_gmg_fp64_matches_oracle(gpu, dims, ndpn, nlevels):
+@pytest.mark.parametrize("dims,ndpn,nlevels,dit", [
+    ((16, 8, 8), 3, 3, 1), ((32, 16, None), 2, 4, 1), ((16, 16, None), 1, 3, 1),
+    ((8, 4, 4), 3, 2, 1),
+    # thin: levels of 3 node planes / lines; the 32x4 strip is so ill-conditioned
+    # (true residual 75x the recursive one at the stop, in the oracle too) that
+    # rounding moves the iteration count by a few (device 36, oracle 39)
+    ((24, 12, 4), 3, 3, 1), ((32, 4, None), 2, 3, 4)])
+def test_pcg_gmg_fp64_matches_oracle(gpu, dims, ndpn, nlevels, dit):
     """With an FP64 V-cycle the device preconditioner is the oracle's V(1,1)
     (same Galerkin hierarchy, omega=0.6, zero coarse guesses, exact coarsest
-    solve), so PCG takes the same number of iterations."""
+    solve): the first PCG iterates agree to rounding, and PCG takes the same
+    number of iterations (within dit)."""
     import torch
     from topoptlab_amd.fem import StiffnessOperator
     say("gmg fp64", dims, ndpn, nlevels)
@@ -124,20 +134,39 @@ def test_pcg_gmg_fp64_matches_oracle(gpu, dims, ndpn, nlevels):
     h = mg.Hierarchy(K, *dims, ndpn, fixed, nlevels)
     u_ref, info, it_ref = mg.pcg(K, b, h.vcycle, rtol=1e-10)
     assert info == 0
+    # both stop on the recursive residual (linear_solvers.py:107-117); on the
+    # ill-conditioned thin grids the true residual then stays above rtol
+    # (oracle: 2.9e-10 on 24x12x4, 7.5e-9 on 32x4), so the device's true
+    # residual is held to the oracle's own
+    tr_ref = max(2e-10, 2 * np.linalg.norm(K @ u_ref - b) / np.linalg.norm(b))
     op = StiffnessOperator(*dims, KE=KE, fixed=fixed, device=gpu)
     op.set_density(x)
     op.setup_mg(nlevels=nlevels, precision="fp64")
+    # the preconditioner itself: iterates 1 and 3 (x1 = alpha M^-1 b, ...)
+    # agree to rounding, which the coarsest solves (dense inverse here, a
+    # sparse direct solve in the oracle) and the PCG recurrences amplify by
+    # the operators' condition (iterate 3: 3.7e-10 on 8x4x4, 1.1e-8 on the
+    # 32x4 strip); mistakes planted in the oracle move them by 3.6e-3 or more
+    # (smoother weight, a missing sweep) and by 8e-6 or more (one missing
+    # interpolation term), test_oracle.py::test_early_pcg_iterates_expose_vcycle_mistakes
+    for k, tol in ((1, 1e-8), (3, 1e-7)):
+        xk_ref, _, _ = mg.pcg(K, b, h.vcycle, rtol=1e-10, maxiter=k)
+        xk, infk = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10, maxiter=k,
+                            raise_on_fail=False)
+        assert infk["iterations"] == k
+        assert rel(xk[op._free_t].cpu().numpy(), xk_ref) < tol, k
     u, inf = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10)
     u = u[op._free_t].cpu().numpy()
-    assert abs(inf["iterations"] - it_ref) <= 1, (inf, it_ref)
+    assert abs(inf["iterations"] - it_ref) <= dit, (inf, it_ref)
     assert rel(u, u_ref) < 1e-7
     ud = spsolve(K.tocsc(), b)
     assert rel(u, ud) < 1e-6
-    assert inf["relres_true"] < 2e-10
-    # FP32 V-cycle converges to the same tolerance in a similar count
+    assert inf["relres_true"] < tr_ref
+    # FP32 V-cycle converges to the same tolerance in a similar count (up to
+    # 30% more on the ill-conditioned thin plate: 65 against 50)
     op.setup_mg(nlevels=nlevels, precision="fp32")
     u32, inf32 = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10)
-    assert inf32["relres_true"] < 2e-10 and inf32["iterations"] <= 1.25 * it_ref + 2
+    assert inf32["relres_true"] < 2 * tr_ref and inf32["iterations"] <= 1.35 * it_ref + 2
     # Jacobi-PCG reaches the same solution
     uj, infj = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10,
                         preconditioner="jacobi", maxiter=200000)

@@ -49,8 +49,10 @@ def segments(nelx, nely, nelz, slots):
             yield hw, bx + 1, 0, rest
 
 
+# the last five: one element, and grids one element thick in z, y or x
+TINY = [(1, 1, 1), (1, 1, 3), (3, 2, 1), (2, 1, 2), (1, 3, 2)]
 DIMS = [(32, 16, 16), (16, 8, 8), (24, 16, 20), (192, 96, 96), (6, 4, 5), (5, 4, 3),
-        (96, 48, 48), (64, 8, 40), (8, 200, 6)]
+        (96, 48, 48), (64, 8, 40), (8, 200, 6)] + TINY
 SLOTS = [512, 768, 1024, 7, 100]
 
 
@@ -97,7 +99,7 @@ def plane_dma(ny, ndpn, size):
 
 
 @pytest.mark.parametrize("ndpn,size", [(3, 8), (3, 4), (1, 8), (1, 4)])
-@pytest.mark.parametrize("dims", [(32, 16, 16), (6, 4, 5), (24, 16, 20), (8, 200, 6)])
+@pytest.mark.parametrize("dims", [(32, 16, 16), (6, 4, 5), (24, 16, 20), (8, 200, 6)] + TINY)
 def test_ku3d_plane_copies_in_bounds(dims, ndpn, size):
     nelx, nely, nelz = dims
     nx, ny, nz = nelx + 1, nely + 1, nelz + 1

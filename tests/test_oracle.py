@@ -572,3 +572,38 @@ def test_reference_interpolator_raises_on_every_grid():
             w = N(x % 2 - 1.0, 1.0 - y % 2)
             got = np.sort(P[x * 5 + y][P[x * 5 + y] != 0])
             assert np.allclose(got, np.sort(w[w != 0])), (x, y)
+
+
+@pytest.mark.parametrize("dims,ndpn,nlevels", [((8, 4, 4), 3, 2), ((32, 4, None), 2, 3)])
+def test_early_pcg_iterates_expose_vcycle_mistakes(dims, ndpn, nlevels):
+    """The device GMG parity test compares the first PCG iterates with the
+    oracle's (iterate 1 to 1e-8, iterate 3 to 1e-7).  Mistakes planted in the
+    oracle's V-cycle move those iterates far beyond that: a smoother weight
+    of 0.55 instead of 0.6, no post-smoothing, and one fine DOF whose
+    interpolation misses one of its coarse terms (Galerkin operators rebuilt
+    to match; scaling a whole column of P would change nothing, since the
+    coarse correction depends only on the range of P)."""
+    from tests.test_gpu_parity import _problem
+    KE, edof, ndof, fixed, free, _ = _problem(dims, ndpn)
+    x = workloads.gaussian_random_density(*dims[:2], dims[2], seed=5, sigma=2.0)
+    K = fem.apply_bc(fem.assemble_K(KE, edof, fem.simp(x), ndof), free).tocsr()
+    b = np.random.default_rng(3).standard_normal(len(free))
+    good = mg.Hierarchy(K, *dims, ndpn, fixed, nlevels)
+    planted = [mg.Hierarchy(K, *dims, ndpn, fixed, nlevels, omega=0.55),
+               mg.Hierarchy(K, *dims, ndpn, fixed, nlevels, nu2=0)]
+    h = mg.Hierarchy(K, *dims, ndpn, fixed, nlevels)
+    P0 = h.P[0].tocsr()
+    P0.eliminate_zeros()
+    i = int(np.flatnonzero(np.diff(P0.indptr) >= 2)[len(b) // 4 % 7])
+    P0 = P0.tolil()
+    P0[i, P0.rows[i][0]] = 0.0
+    h.P[0] = P0.tocsr()
+    for lv in range(len(h.P)):
+        h.A[lv + 1] = (h.P[lv].T @ h.A[lv] @ h.P[lv]).tocsr()
+    h.Dinv = [1.0 / A.diagonal() for A in h.A]
+    planted.append(h)
+    for k, tol in ((1, 1e-8), (3, 1e-7)):
+        ref, _, _ = mg.pcg(K, b, good.vcycle, rtol=1e-10, maxiter=k)
+        for bad in planted:
+            xk, _, _ = mg.pcg(K, b, bad.vcycle, rtol=1e-10, maxiter=k)
+            assert rel(xk, ref) > 10 * tol, (k, rel(xk, ref))

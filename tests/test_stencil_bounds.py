@@ -49,7 +49,8 @@ def term_offsets(nx, ny, nz, ndpn, k):
     return xo, bo, ok, m
 
 
-@pytest.mark.parametrize("dims", [(2, 2, 2), (5, 3, 4), (9, 5, 5), (3, 17, 2), (33, 17, 17)])
+@pytest.mark.parametrize("dims", [(2, 2, 2), (5, 3, 4), (9, 5, 5), (3, 17, 2), (33, 17, 17),
+                                  (13, 7, 3), (7, 4, 2)])
 @pytest.mark.parametrize("ndpn", [1, 3])
 def test_group_form_buffer_loads_in_bounds(dims, ndpn):
     nx, ny, nz = dims
