@@ -298,8 +298,11 @@ def test_pcg_wcycle_fp64_matches_oracle(gpu, gamma_from):
     op.close()
 
 
-@pytest.mark.parametrize("dims", [(60, 20, None), (13, 7, None), (10, 8, 6)])
-@pytest.mark.parametrize("rmin", [2.4, 1.2, 3.0])
+# radii below one element (the element itself only) and wider than the grid;
+# a one-element grid and a thin 3D grid
+@pytest.mark.parametrize("dims", [(60, 20, None), (13, 7, None), (10, 8, 6), (1, 1, None),
+                                  (4, 3, 2)])
+@pytest.mark.parametrize("rmin", [2.4, 1.2, 3.0, 0.7, 5.5])
 def test_filter_bitexact(gpu, dims, rmin):
     import torch
     from topoptlab_amd.filter import assemble_matrix_filter
