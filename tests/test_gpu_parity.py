@@ -325,7 +325,8 @@ def test_filter_bitexact(gpu, dims, rmin):
 
 
 @pytest.mark.parametrize("dims,ndpn,ncases", [((12, 8, None), 2, 2), ((6, 4, 5), 3, 1),
-                                              ((10, 6, None), 1, 1)])
+                                              ((10, 6, None), 1, 1), ((1, 1, None), 2, 2),
+                                              ((1, 1, 1), 3, 1), ((3, 1, 2), 1, 3)])
 def test_compliance_sens(gpu, dims, ndpn, ncases):
     import torch
     from topoptlab_amd.fem import StiffnessOperator
@@ -340,7 +341,9 @@ def test_compliance_sens(gpu, dims, ndpn, ncases):
     op.close()
 
 
-@pytest.mark.parametrize("n,g", [(1200, 0.0), (5000, 0.3), (70000, -0.01)])
+# one element, and more elements than the bisection grid has threads (1024 x 256)
+@pytest.mark.parametrize("n,g", [(1200, 0.0), (5000, 0.3), (70000, -0.01), (1, 0.0),
+                                 (300000, 0.05)])
 def test_oc_bitexact(gpu, n, g):
     import torch
     from topoptlab_amd.optimizer import oc_top88
