@@ -8,7 +8,8 @@
   the same iteration counts and bit-identical solutions as one-at-a-time
   ``solve`` calls, with graphs on, off, and on again (cached graphs
   replayed), and its solutions meet the tolerance against the oracle's
-  assembled matrix.
+  assembled matrix; also for a batch of one instance with a 1-D right-hand
+  side and a batch mixing 2D and 3D instances.
 """
 import sys
 
@@ -196,3 +197,42 @@ def test_batch_reports_non_convergence(gpu):
                 assert inf["status"] != 0 and inf["iterations"] <= 4
     for op in ops:
         op.close()
+
+
+def test_batch_single_instance_and_mixed_dimensions(gpu):
+    """A batch of one instance with a one-dimensional right-hand side, and a
+    batch that mixes a 2D and a 3D instance (not eligible for lockstep, so
+    the per-instance streams run it even when lockstep is asked for): both
+    equal the single solves bit for bit."""
+    import torch
+    from topoptlab_amd.batch import batch_solve, build_instance
+    from topoptlab_amd.fem import StiffnessOperator
+    from topoptlab_amd.workloads import batched_instances, gaussian_random_density
+    from tests.test_gpu_parity import _problem
+    spec = batched_instances(seed=6, n_elast=1, n_heat=0, xrange=(64, 96), yrange=(32, 64),
+                             ncases=2)[0]
+    op2, x2, f2, lv2 = build_instance(spec, gpu)
+    op2.set_density(x2)
+    op2.setup_mg(nlevels=lv2, precision="fp32")
+    f2 = torch.as_tensor(f2, device=gpu)
+    KE, edof, ndof, fixed, free, _ = _problem((8, 4, 4), 3)
+    op3 = StiffnessOperator(8, 4, 4, KE=KE, fixed=fixed, device=gpu)
+    op3.set_density(gaussian_random_density(8, 4, 4, seed=5, sigma=2.0))
+    op3.setup_mg(nlevels=2, precision="fp32")
+    f3 = torch.zeros(ndof, dtype=torch.float64, device=gpu)
+    f3[torch.as_tensor(free, device=gpu)] = torch.as_tensor(
+        np.random.default_rng(7).standard_normal(len(free)), device=gpu)
+    ref2 = op2.solve(f2[0], rtol=1e-8)
+    ref3 = op3.solve(f3, rtol=1e-8)
+    for lockstep in (True, False):
+        say("batch of one, 1-D right-hand side, lockstep", lockstep)
+        U, infos = batch_solve([op2], [f2[0]], rtol=1e-8, chunk=4, lockstep=lockstep)
+        assert U[0].shape == f2[0].shape and len(infos[0]) == 1
+        assert infos[0][0]["iterations"] == ref2[1]["iterations"]
+        assert torch.equal(U[0], ref2[0])
+        say("mixed 2D and 3D batch, lockstep", lockstep)
+        U, infos = batch_solve([op2, op3], [f2[0], f3], rtol=1e-8, chunk=4, lockstep=lockstep)
+        assert torch.equal(U[0], ref2[0]) and torch.equal(U[1], ref3[0])
+        assert infos[1][0]["iterations"] == ref3[1]["iterations"]
+    op2.close()
+    op3.close()
