@@ -557,15 +557,16 @@ def test_fp64_mfma_operator_matches_vector(gpu, dims):
     op.close()
 
 
-@pytest.mark.parametrize("it", [0, 5])
-def test_mma_update_matches_oracle(gpu, it):
+# both asymptote branches; one design variable, and more variables than the
+# reduction grid has threads (1024 x 256)
+@pytest.mark.parametrize("it,n", [(0, 3000), (5, 3000), (0, 1), (5, 1), (5, 300000)])
+def test_mma_update_matches_oracle(gpu, it, n):
     """tl_mma_update against the oracle's mmasub: both asymptote branches."""
     import torch
     from oracle import mma
     from topoptlab_amd.optimizer import mma_defaultkws, mma_update
-    say("mma update", it)
+    say("mma update", it, n)
     rng = np.random.default_rng(it)
-    n = 3000
     x = 0.05 + 0.9 * rng.random(n)
     xold1 = np.clip(x + 0.05 * rng.standard_normal(n), 0.0, 1.0)
     xold2 = np.clip(xold1 + 0.05 * rng.standard_normal(n), 0.0, 1.0)
