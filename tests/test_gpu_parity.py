@@ -399,7 +399,9 @@ def test_main_helmholtz_golden(gpu, kats, ft, ref_idx):
     assert obj == pytest.approx(ref, abs=1.5e-5)
 
 
-@pytest.mark.parametrize("nelx,nely", [(60, 20), (37, 13)])
+# 2 GMG levels, Jacobi (odd sizes), one element, a strip two elements wide,
+# and 4 GMG levels
+@pytest.mark.parametrize("nelx,nely", [(60, 20), (37, 13), (1, 1), (2, 64), (256, 128)])
 def test_helmholtz_filter_matches_oracle(gpu, nelx, nely):
     import torch
     from scipy.sparse.linalg import spsolve
