@@ -271,26 +271,32 @@ def test_c2_solve_matches_oracle_fixture(gpu):
     op.close()
 
 
+@pytest.mark.parametrize("dims,ndpn,nlevels", [((32, 16, 16), 3, 4), ((64, 32, None), 2, 4)])
 @pytest.mark.parametrize("gamma_from", [1, 2])
-def test_pcg_wcycle_fp64_matches_oracle(gpu, gamma_from):
-    """FP64 W-cycle (gamma = 2 from level gamma_from): same iteration count
-    as the oracle's W-cycle preconditioner, same solution."""
+def test_pcg_wcycle_fp64_matches_oracle(gpu, gamma_from, dims, ndpn, nlevels):
+    """FP64 W-cycle (gamma = 2 from level gamma_from): the first PCG iterate
+    (alpha times the W-cycle applied to b) equals the oracle's, and PCG takes
+    the same iteration count as the oracle's W-cycle preconditioner, with
+    the same solution."""
     import torch
     from topoptlab_amd.fem import StiffnessOperator
-    dims, nlevels = (32, 16, 16), 4
     say("gmg fp64 W-cycle", dims, nlevels, gamma_from)
-    KE, edof, ndof, fixed, free, _ = _problem(dims, 3)
-    x = workloads.gaussian_random_density(*dims, seed=5, sigma=2.0)
+    KE, edof, ndof, fixed, free, _ = _problem(dims, ndpn)
+    x = workloads.gaussian_random_density(*dims[:2], dims[2], seed=5, sigma=2.0)
     K = fem.apply_bc(fem.assemble_K(KE, edof, fem.simp(x), ndof), free).tocsr()
     b = np.random.default_rng(3).standard_normal(len(free))
-    h = mg.Hierarchy(K, *dims, 3, fixed, nlevels)
-    u_ref, info, it_ref = mg.pcg(K, b, lambda r: h.vcycle(r, gamma=2, gamma_from=gamma_from),
-                                 rtol=1e-10)
+    h = mg.Hierarchy(K, *dims, ndpn, fixed, nlevels)
+    wcyc = lambda r: h.vcycle(r, gamma=2, gamma_from=gamma_from)
+    u_ref, info, it_ref = mg.pcg(K, b, wcyc, rtol=1e-10)
     _, _, it_v = mg.pcg(K, b, h.vcycle, rtol=1e-10)
     assert info == 0 and it_ref <= it_v
     op = StiffnessOperator(*dims, KE=KE, fixed=fixed, device=gpu)
     op.set_density(x)
     op.setup_mg(nlevels=nlevels, precision="fp64", gamma=2, gamma_from=gamma_from)
+    x1_ref, _, _ = mg.pcg(K, b, wcyc, rtol=1e-10, maxiter=1)
+    x1, _ = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10, maxiter=1,
+                     raise_on_fail=False)
+    assert rel(x1[op._free_t].cpu().numpy(), x1_ref) < 1e-8
     u, inf = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10)
     say("W-cycle iterations", inf["iterations"], "oracle", it_ref, "V", it_v)
     assert abs(inf["iterations"] - it_ref) <= 1, (inf, it_ref)
