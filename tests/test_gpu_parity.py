@@ -146,9 +146,9 @@ def test_pcg_gmg_fp64_matches_oracle(gpu, dims, ndpn, nlevels, dit):
     # agree to rounding, which the coarsest solves (dense inverse here, a
     # sparse direct solve in the oracle) and the PCG recurrences amplify by
     # the operators' condition (iterate 3: 3.7e-10 on 8x4x4, 1.1e-8 on the
-    # 32x4 strip); mistakes planted in the oracle move them by 3.6e-3 or more
-    # (smoother weight, a missing sweep) and by 8e-6 or more (one missing
-    # interpolation term), test_oracle.py::test_early_pcg_iterates_expose_vcycle_mistakes
+    # 32x4 strip); mistakes planted in the oracle (smoother weight, a missing
+    # sweep, a missing interpolation term) move them by 3.6e-3 or more,
+    # test_oracle.py::test_early_pcg_iterates_expose_vcycle_mistakes
     for k, tol in ((1, 1e-8), (3, 1e-7)):
         xk_ref, _, _ = mg.pcg(K, b, h.vcycle, rtol=1e-10, maxiter=k)
         xk, infk = op.solve(op.full(torch.as_tensor(b, device=gpu)), rtol=1e-10, maxiter=k,
