@@ -557,6 +557,82 @@ __global__ __launch_bounds__(BS) void st_apply_kernel(GridD g, const T* __restri
   st_apply_node<NDPN, T, MODE, DIMU, PRO>(g, sth, x, b, dinv, omega, mask, y, n, cc);
 }
 
+// Small 3D levels (up to 16,384 nodes): four lanes per node.  Lane q of a
+// node's quad loads and sums the terms K = q, q + 4,
+// ... of its 27-term row (runtime offsets, so the quad's lanes run one
+// instruction stream), all of its loads issued before the first sum; the
+// quad's partial sums are then combined by two butterfly exchanges (every
+// lane ends with the same total) and lanes 0..NDPN-1 write one DOF each.
+// Four times the threads of the one-lane form and a quarter of its loads per
+// lane: these levels have too few waves to hide a chain of dependent loads.
+// On larger levels (30,625 nodes) the one-lane form with three pairs per
+// load group is faster: it already has waves enough, and the quad's extra
+// threads and exchanges cost more than the shorter chains save.
+template <int NDPN, typename T, int MODE>
+__global__ __launch_bounds__(64) void st_apply_quad_kernel(GridD g, const T* __restrict__ sth,
+                                                           const T* __restrict__ x,
+                                                           const T* __restrict__ b,
+                                                           const T* __restrict__ dinv, T omega,
+                                                           const uint8_t* __restrict__ mask,
+                                                           T* __restrict__ y, const int* done) {
+  constexpr int NN = NDPN * NDPN, NT = 7;  // terms per lane (lane 3: 6)
+  if (*done) return;
+  const unsigned nb = gridDim.x, hw = blockIdx.x;
+  const unsigned xcd = hw & 7u, qq = nb >> 3, rr = nb & 7u;
+  const unsigned logical = xcd * qq + min(xcd, rr) + (hw >> 3);
+  const int64_t n = (int64_t(logical) * 64 + threadIdx.x) >> 2;
+  const int q = int(threadIdx.x & 3u);
+  if (n >= g.nnodes) return;  // a whole quad: its four lanes share n
+  int ix, iy, iz;
+  node_xyz(g, n, ix, iy, iz);
+  T xm[NT][NDPN], w[NT][NN];
+#pragma unroll
+  for (int j = 0; j < NT; ++j) {
+    const int K = min(q + 4 * j, 26);  // lane 3's seventh slot repeats term 26 with weight 0
+    const bool live = q + 4 * j <= 26;
+    const int rz = K / 9 - 1, rx = K / 3 % 3 - 1, ry = K % 3 - 1;
+    const int64_t m = node_id(g, min(max(ix + rx, 0), g.nx - 1), min(max(iy + ry, 0), g.ny - 1),
+                              min(max(iz + rz, 0), g.nz - 1));
+#pragma unroll
+    for (int c = 0; c < NDPN; ++c) xm[j][c] = x[m * NDPN + c];
+    // forward block of n (K >= 13), else the transposed forward block of m
+    const bool fwd = K >= 13;
+    const bool ok = live && (fwd || node_in(g, ix + rx, iy + ry, iz + rz));
+    const T* h = sth + int64_t(fwd ? K - 13 : 13 - K) * NN * g.nnodes + (fwd ? n : m);
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+      for (int c = 0; c < NDPN; ++c) {
+        const T v = h[int64_t(fwd ? d * NDPN + c : c * NDPN + d) * g.nnodes];
+        w[j][d * NDPN + c] = ok ? v : T(0);
+      }
+  }
+  __builtin_amdgcn_sched_barrier(0);
+  T acc[NDPN];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) acc[d] = T(0);
+#pragma unroll
+  for (int j = 0; j < NT; ++j)
+#pragma unroll
+    for (int d = 0; d < NDPN; ++d)
+#pragma unroll
+      for (int c = 0; c < NDPN; ++c) acc[d] += w[j][d * NDPN + c] * xm[j][c];
+#pragma unroll
+  for (int d = 0; d < NDPN; ++d) {
+    acc[d] += __shfl_xor(acc[d], 1, 64);
+    acc[d] += __shfl_xor(acc[d], 2, 64);
+  }
+  if (q >= NDPN) return;
+  T a = acc[0];
+#pragma unroll
+  for (int d = 1; d < NDPN; ++d) a = q == d ? acc[d] : a;
+  const int64_t i = n * NDPN + q;
+  T v;
+  if (MODE == 0) v = b[i] - a;
+  else v = x[i] + omega * dinv[i] * (b[i] - a);
+  y[i] = ((mask[n] >> q) & 1) ? T(0) : v;
+}
+
 // x = omega*dinv*b  (first smoothing sweep from a zero guess); b may be fp64
 template <typename T, typename TB>
 __global__ void jacobi_zero_kernel(const TB* __restrict__ b, const T* __restrict__ dinv, T omega,
@@ -1373,6 +1449,18 @@ int st_apply(tl_problem* P, int l, int mode, const T* x, const T* b, T* y, const
   using I5 = std::integral_constant<int, 5>;
   if (dimu == 4) {
     if constexpr (NDPN != 2) { if (mode == 0) go(I0{}, I4{}); else go(I1{}, I4{}); }
+  } else if (dimu == 5 && !P->rec && L.g.nnodes <= 16384) {
+    // four lanes per node on the smallest levels (3D levels are never
+    // recorded for a lockstep batch)
+    if constexpr (NDPN != 2) {
+      const unsigned nq = unsigned((4 * L.g.nnodes + kStBlock3d - 1) / kStBlock3d);
+      if (mode == 0)
+        hipLaunchKernelGGL((st_apply_quad_kernel<NDPN, T, 0>), dim3(nq), dim3(kStBlock3d), 0,
+                           P->stream, gd, st, x, b, dinv, om, mk, y, done);
+      else
+        hipLaunchKernelGGL((st_apply_quad_kernel<NDPN, T, 1>), dim3(nq), dim3(kStBlock3d), 0,
+                           P->stream, gd, st, x, b, dinv, om, mk, y, done);
+    }
   } else if (dimu == 5) {
     if constexpr (NDPN != 2) { if (mode == 0) go(I0{}, I5{}); else go(I1{}, I5{}); }
   } else if (dimu == 2) {
