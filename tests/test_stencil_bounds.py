@@ -82,3 +82,46 @@ def test_size_rule_keeps_offsets_in_32_bits():
         # the largest voffset + soffset a load forms stays below the slab size
         assert (nmax - 1) * 8 + (ndpn * ndpn - 1) * nmax * 8 < 1 << 32
         assert (nmax - 1) * ndpn * 8 + (ndpn - 1) * 8 < 1 << 32
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2), (5, 3, 4), (13, 7, 7), (25, 13, 13), (3, 17, 2)])
+@pytest.mark.parametrize("ndpn", [1, 3])
+def test_lane_form_covers_each_term_once_in_bounds(dims, ndpn):
+    """st_apply_quad_kernel (four lanes per node, 64-thread workgroups, the
+    XCD-aware block order; lane q sums the terms K = q, q + 4, ... and writes
+    DOF q): the live terms of a node's lanes are its 27 terms exactly once,
+    every DOF is written by exactly one lane, and every read (x of the
+    clamped neighbour, the forward-half block) is inside its buffer."""
+    lanes = 4
+    nx, ny, nz = dims
+    nnodes = nx * ny * nz
+    nn, nt = ndpn * ndpn, 7
+    ng = -(-27 // (lanes * nt))
+    nb = -(-lanes * nnodes // 64)
+    t = logical_blocks(nb)[:, None] * 64 + np.arange(64)[None, :]
+    n, q = (t // lanes).ravel(), (t % lanes).ravel()
+    keep = n < nnodes
+    n, q = n[keep], q[keep]
+    iy, r = n % ny, n // ny
+    ix, iz = r % nx, r // nx
+    terms = np.zeros((nnodes, 27), dtype=np.int64)
+    for j in range(ng * nt):
+        kraw = q + lanes * j
+        live = kraw <= 26
+        k = np.minimum(kraw, 26)
+        rz, rx, ry = k // 9 - 1, k // 3 % 3 - 1, k % 3 - 1
+        m = (np.clip(iz + rz, 0, nz - 1) * nx + np.clip(ix + rx, 0, nx - 1)) * ny + \
+            np.clip(iy + ry, 0, ny - 1)
+        assert m.min() >= 0 and (m * ndpn + ndpn - 1).max() < nnodes * ndpn
+        fwd = k >= 13
+        slab = np.where(fwd, k - 13, 13 - k)
+        base = np.where(fwd, n, m)
+        idx = slab * nn * nnodes + (nn - 1) * nnodes + base
+        assert idx.min() >= 0 and idx.max() < 14 * nn * nnodes
+        np.add.at(terms, (n[live], k[live]), 1)
+    assert (terms == 1).all()
+    writes = np.zeros((nnodes, ndpn), dtype=np.int64)
+    for d in range(ndpn):
+        sel = d % lanes == q
+        np.add.at(writes, (n[sel], d), 1)
+    assert (writes == 1).all()
